@@ -1,0 +1,297 @@
+/*
+ * semblance.h -- C ABI of libsemblance, the MI355X (gfx950) CMP / CRS / CRS-DE
+ * semblance search.
+ *
+ * The reference ships no library: its seams are the OpenMP loops and the CUDA
+ * kernels inside each CLI program.  Every entry point below names the
+ * reference interface it replaces (paths relative to the reference tree).
+ *
+ * Conventions
+ *   - Plain C types only.  Every call returns an int status (SB_OK = 0);
+ *     sb_last_error() gives a thread-local detail string.  Nothing exits the
+ *     process (the reference exits on every error: CMP/OpenMP/src/log.cpp:44-46).
+ *   - Host arrays are caller-owned.  Device memory is owned by an sb_ctx and
+ *     the objects created from it.
+ *   - One sb_ctx per host thread.  Calls on one ctx are serialised on the
+ *     ctx's HIP stream.
+ *   - Result arrays are [ncdp][ns], CDP-major, in the gather's CDP order.
+ */
+#ifndef SEMBLANCE_H
+#define SEMBLANCE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+  SB_OK = 0,
+  SB_ERR_INVALID = 1,     /* bad argument or inconsistent sizes            */
+  SB_ERR_IO = 2,          /* file missing, short read ("Bad input data")   */
+  SB_ERR_FORMAT = 3,      /* mixed ns in one file, empty gather, ...       */
+  SB_ERR_HIP = 4,         /* HIP runtime error / no device                 */
+  SB_ERR_BUDGET = 5,      /* request exceeds the ctx HBM budget            */
+  SB_ERR_NOMEM = 6,       /* host allocation failed                        */
+  SB_ERR_UNSUPPORTED = 7, /* e.g. window w = 2*tau+1 above SB_MAX_W        */
+  SB_ERR_STATE = 8        /* object used after shutdown / wrong method     */
+};
+
+#define SB_MAX_W 15       /* largest coherence window (2*tau+1) supported   */
+
+const char* sb_strerror(int code);
+const char* sb_last_error(void);
+const char* sb_version(void);
+
+/* ------------------------------------------------------------ enumerations */
+enum { SB_METHOD_CMP = 0, SB_METHOD_CRS = 1 };
+
+/* Device storage of trace amplitudes.  F16/BF16 keep a per-trace power-of-two
+ * scale; amplitudes are restored in FP32 before any accumulation. */
+enum { SB_STORE_F32 = 0, SB_STORE_F16 = 1, SB_STORE_BF16 = 2 };
+
+/* Arithmetic of the search.
+ *  SB_MODE_FAST : FP32 search (contracted FMAs, hardware sqrt, window sums
+ *                 reassociated); the winning parameter of every (CDP, t0) is
+ *                 then re-evaluated in the reference's precision and
+ *                 operation order (CMP float, CRS double).
+ *  SB_MODE_EXACT: the whole search in the reference's precision and
+ *                 operation order (CMP float, CRS double), no contraction,
+ *                 correctly rounded sqrt/div.  Bit-identical to the
+ *                 reference OpenMP programs on the same F32 input. */
+enum { SB_MODE_FAST = 0, SB_MODE_EXACT = 1 };
+
+/* --------------------------------------------------- SU files (host side) */
+/* One trace's header fields used by the path, at their SU byte offsets. */
+typedef struct {
+  int32_t cdp;    /* @20  */
+  int32_t scalco; /* @70 (short in the file) */
+  int32_t sx;     /* @72  */
+  int32_t sy;     /* @76  */
+  int32_t gx;     /* @80  */
+  int32_t gy;     /* @84  */
+} sb_trace_geom;
+
+typedef struct sb_su_gather sb_su_gather; /* a binned SU file in host memory */
+
+/* Reads a native-endian SU file, drops traces whose half-offset exceeds aph,
+ * bins by the cdp header and orders CDPs like the reference:
+ *   SB_METHOD_CMP: CDPs sorted by fold, descending (std::sort)
+ *                  -- replaces su_gather::su_gather, CMP/OpenMP/src/su_gather.cpp:33-69
+ *   SB_METHOD_CRS: CDPs by ascending key, plus for each CDP the list of CDPs
+ *                  whose first-trace midpoint lies within apm
+ *                  -- replaces CRS/OpenMP/src/su_gather.cpp:33-90
+ * and linearizes samples to [ntr][ns] (su_gather::linearize,
+ * CMP/OpenMP/src/su_gather.cpp:73-106). */
+int sb_su_load(const char* path, int method, int aph, int apm, sb_su_gather** out);
+void sb_su_free(sb_su_gather* g);
+
+/* Sizes of a loaded gather.  nnbr = total neighbour-list length (CRS). */
+int sb_su_info(const sb_su_gather* g, int* ncdp, int* ntr, int* ns, int* dt_us, int* nnbr);
+/* Borrowed views, valid until sb_su_free. */
+const int32_t* sb_su_cdp_offsets(const sb_su_gather* g);   /* [ncdp+1] */
+const float* sb_su_samples(const sb_su_gather* g);         /* [ntr][ns] */
+const sb_trace_geom* sb_su_geom(const sb_su_gather* g);    /* [ntr] */
+const int32_t* sb_su_nbr_offsets(const sb_su_gather* g);   /* [ncdp+1] or NULL */
+const int32_t* sb_su_nbr(const sb_su_gather* g);           /* [nnbr] or NULL */
+
+/* Writes one output section: one trace per CDP carrying the header of the
+ * CDP's first trace and data = values[cdp][0..ns).  With rewrite_header = 1
+ * (parameter sections: CMP c, CRS a/b/c) the header gets offset = 0,
+ * sx = gx = (sx+gx)>>1, sy = gy = (sy+gy)>>1; coherence and stack sections
+ * (rewrite_header = 0) keep it as read.  Replaces the output loops
+ * CMP/OpenMP/src/main.cpp:244-260 and CRS/OpenMP/src/main.cpp:331-359. */
+int sb_su_write_section(const sb_su_gather* g, const char* path, const float* values, int rewrite_header);
+
+/* Writes traces with a zeroed 240-byte header carrying the given fields plus
+ * ns, dt (us), tracl/tracr = 1-based trace number and offset = the signed
+ * source-receiver distance in header units (su_trace::fputtr,
+ * CMP/OpenMP/src/su_trace.cpp:55-66). */
+int sb_su_write_traces(const char* path, int ntr, int ns, int dt_us,
+                       const sb_trace_geom* geom, const float* samples);
+
+/* Reference neighbour lists (CRS/OpenMP/src/su_gather.cpp:68-87) computed
+ * from the first trace of each CDP.  Call with nbr = NULL to get *nnbr. */
+int sb_crs_neighbors(int ncdp, const sb_trace_geom* first_geom, int apm,
+                     int32_t* nbr_off, int32_t* nbr, int* nnbr);
+
+/* Keep-test of the aph aperture exactly as su_gather applies it: the
+ * half-offset from su_trace::halfoffset_x/y (float), squared and summed in
+ * float for CMP (CMP/OpenMP/src/su_gather.cpp:44-47) and in double for CRS
+ * (CRS/OpenMP/src/su_gather.cpp:215-218), against aph*aph.  Returns 1 = keep. */
+int sb_keep_trace(const sb_trace_geom* g, int method, int aph);
+
+/* ------------------------------------------------------- device context */
+typedef struct sb_ctx sb_ctx;
+
+/* hbm_budget = bytes this ctx may hold on the device (0 = 90% of free HBM). */
+int sb_ctx_create(int device, size_t hbm_budget, sb_ctx** out);
+void sb_ctx_destroy(sb_ctx* ctx);
+/* Bytes currently held / budget. */
+int sb_ctx_memory(const sb_ctx* ctx, size_t* used, size_t* budget);
+/* Waits for all work queued on the ctx stream. */
+int sb_ctx_sync(sb_ctx* ctx);
+
+/* ---------------------------------------------- device-resident gathers */
+typedef struct sb_gather sb_gather;
+
+/* Uploads a binned gather (rows [cdp_off[i], cdp_off[i+1]) belong to CDP i),
+ * computes per-trace geometry on the device (FP64, then stored FP32, or the
+ * reference's own float/double arithmetic for SB_MODE_EXACT searches) and,
+ * for CRS (nbr_off != NULL), the supergather slot table of every CDP.
+ * samples: [ntr][ns] FP32, quantised on the device when store != F32.
+ * Replaces the setup of CMP/CUDA/src/main.cu:232-274 (init_half) and
+ * CRS/CUDA/src/main.cu:407-434 (init_mid, compute_points_for_gather). */
+int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
+                     const sb_trace_geom* geom, const float* samples, int store,
+                     const int32_t* nbr_off, const int32_t* nbr, sb_gather** out);
+void sb_gather_destroy(sb_gather* g);
+int sb_gather_info(const sb_gather* g, int* ncdp, int* ntr, int* ns, int* store, size_t* bytes);
+
+/* ------------------------------------------------------------- searches */
+typedef struct {
+  double c0, c1;  /* search range of C in s^2/m^2, as parsed (-c0/-c1)       */
+  int nc;         /* number of C values; grid excludes c1 (Q1)              */
+  double tau;     /* half window in seconds (-tau)                          */
+  int mode;       /* SB_MODE_*                                              */
+} sb_cmp_params;
+
+typedef struct {
+  double a0, a1, b0, b1, c0, c1; /* -a0 -a1 -b0 -b1 -c0 -c1                */
+  int na, nb, nc;                /* -na -nb -nc                            */
+  double tau;                    /* -tau                                   */
+  int mode;                      /* SB_MODE_*                              */
+} sb_crs_params;
+
+typedef struct {
+  double a0, a1, b0, b1, c0, c1; /* search box, as for CRS                  */
+  double tau;
+  int pop;        /* population per (CDP, t0), 4..64                         */
+  int ngen;       /* generations (-ngen)                                     */
+  double F, CR;   /* differential weight and crossover rate                 */
+  uint64_t seed;
+  int mode;       /* SB_MODE_FAST: FP32 contracted; SB_MODE_EXACT: FP32 in
+                     the oracle's operation order (bit-identical CPU twin)  */
+} sb_de_params;
+
+/* Per-(CDP, t0) results.  index: CMP c index (-1: none), CRS flat
+ * (ia*nb+ib)*nc+ic, DE -1.  a/b/c: parameter values written to the
+ * reference's output files (CMP: c only, in s^2/m^2). */
+typedef struct {
+  int32_t* index;
+  float* a;
+  float* b;
+  float* c;
+  float* coher;
+  float* stack;
+} sb_host_result;
+
+typedef struct sb_result sb_result; /* device-resident result arrays */
+int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out);
+void sb_result_destroy(sb_result* r);
+/* Copies a result to host arrays (any pointer may be NULL to skip it). */
+int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_result* out);
+
+/* CMP search: replaces compute_semblances + redux_semblances,
+ * CMP/CUDA/src/main.cu:87-173 (OpenMP: CMP/OpenMP/src/main.cpp:119-222). */
+int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_result* r);
+/* CRS grid search: replaces compute_points_for_gather + compute_semblances +
+ * redux_semblances, CRS/CUDA/src/main.cu:114-264 (OpenMP: CRS/OpenMP/src/main.cpp:155-285). */
+int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_result* r);
+/* CRS-DE search (described only in README.md:82-131 of the reference). */
+int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_result* r);
+
+/* Convenience: search and download in one call. */
+int sb_cmp(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_host_result* out);
+int sb_crs(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_host_result* out);
+int sb_crs_de(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_host_result* out);
+
+/* Timing of the last *_run on this ctx, from HIP events recorded on the ctx
+ * stream around each kernel.  tuples = sum over CDPs of traces x ns x params
+ * (the reference's "Semblances-Trace" count). */
+typedef struct {
+  double search_ms;    /* main search kernel                     */
+  double finalize_ms;  /* winner re-evaluation / output kernel   */
+  double total_ms;     /* first to last event of the call        */
+  double tuples;
+  int grid_blocks;     /* main kernel launch size                */
+  int block_threads;
+} sb_timing;
+int sb_last_timing(const sb_ctx* ctx, sb_timing* t);
+
+/* Launch shape of the SB_MODE_FAST search on this ctx: variant indexes the
+ * compiled (t0 rows per lane, parameters per pass) shapes; pchunk = parameters
+ * per workgroup (0 = automatic).  sb_tune measures every shape on the given
+ * gather and keeps the fastest (the project's autotuner). */
+int sb_tune_set(sb_ctx* ctx, int variant, int pchunk);
+int sb_tune_get(const sb_ctx* ctx, int* variant, int* pchunk, int* nvariants);
+int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int reps);
+
+/* ------------------------------------------------------ synthetic data */
+/* Seeded synthetic gathers for the benchmark configurations (host, OpenMP).
+ * kind 0: 2D line, CMP events t^2 = t0^2 + C h^2 (h = half-offset);
+ * kind 1: 3D-style CDPs, CRS events t^2 = (t0 + A dm)^2 + t0 B dm^2 + C h^2. */
+typedef struct {
+  int kind;
+  int ncdp;
+  int fold_min, fold_max;   /* uniform integer fold per CDP            */
+  int ns, dt_us;
+  double cdp_spacing;       /* m                                       */
+  double h_max;             /* half-offsets in [0, h_max] m            */
+  double h_jitter;          /* kind 0: +- jitter on the regular offsets */
+  int regular_offsets;      /* kind 0: 1 = h_j = j * h_step            */
+  double h_step;            /* kind 0 regular step                     */
+  double mid_half_width;    /* kind 1: midpoints uniform +- this (x,y) */
+  int nevents;
+  double freq;              /* Ricker peak frequency, Hz               */
+  double snr;               /* 0 = no noise                            */
+  int aph;                  /* keep only traces passing the aph test (0 = keep all) */
+  uint64_t seed;
+} sb_synth_spec;
+
+/* Two-call pattern: samples == NULL returns the kept trace count in *ntr and
+ * the CDP count in *ncdp; then pass arrays of that size.  Traces come out
+ * binned by CDP in ascending key order with cdp_off[ncdp+1]. */
+int sb_synth(const sb_synth_spec* spec, int* ncdp, int* ntr, int32_t* cdp_off,
+             sb_trace_geom* geom, float* samples);
+
+/* ------------------------------------------------------------- serving */
+/* A resident request loop for interactive velocity analysis: requests of
+ * uneven fold / length are admitted while the HBM budget allows, their
+ * gathers stay on the device, and admitted requests are batched per launch. */
+typedef struct sb_server sb_server;
+
+typedef struct {
+  int method;              /* SB_METHOD_CMP or SB_METHOD_CRS               */
+  int ncdp, ns, dt_us;
+  const int32_t* cdp_off;  /* [ncdp+1]                                     */
+  const sb_trace_geom* geom;
+  const float* samples;    /* [ntr][ns]                                    */
+  const int32_t* nbr_off;  /* CRS only                                     */
+  const int32_t* nbr;
+  sb_cmp_params cmp;
+  sb_crs_params crs;
+  int store;
+} sb_request;
+
+typedef struct {
+  double t_submit_ms, t_admit_ms, t_done_ms; /* server clock                */
+  int status;
+} sb_request_stats;
+
+int sb_server_create(int device, size_t hbm_budget, int max_batch, sb_server** out);
+/* Copies the request; returns a ticket.  Never blocks on the GPU. */
+int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* ticket);
+/* Blocks until the ticket is done (timeout_ms < 0: forever); copies results
+ * into out (arrays [ncdp][ns]) and releases the request. */
+int sb_server_wait(sb_server* s, uint64_t ticket, double timeout_ms,
+                   sb_host_result* out, sb_request_stats* st);
+int sb_server_stats(const sb_server* s, uint64_t* done, uint64_t* launches, size_t* hbm_used);
+void sb_server_destroy(sb_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEMBLANCE_H */

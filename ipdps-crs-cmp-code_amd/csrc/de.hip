@@ -1,0 +1,323 @@
+// CRS-DE: differential evolution over the CRS (A, B, C) box per (CDP, t0).
+//
+// The reference only describes this method (README.md:82-131: binary
+// crs-ocl-de, flags -ngen/-azimuth); the algorithm is DE/rand/1/bin
+// (Storn & Price 1997) with the CRS semblance of CRS/OpenMP/src/main.cpp:218-265
+// as fitness (invalid -> 0), a synchronous generation update, clipping to the
+// box and ">=" selection.  Random draws are a counter-based hash of
+// (seed, CDP index, t0, generation, member, draw), so the CPU twin draws the
+// same numbers.
+//
+// Mapping: one lane per population member; a segment of P2 = pow2 >= pop
+// lanes holds one (CDP, t0), so a wave carries 64/P2 consecutive t0.  Each
+// lane walks the CDP's supergather serially for its own candidate; the
+// mutation reads r1/r2/r3 from sibling lanes with ds_bpermute.
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace sb {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t de_hash(uint64_t seed, uint32_t cdp, uint32_t t0, uint32_t gen, uint32_t member,
+                                            uint32_t draw) {
+  uint64_t k = mix64(seed ^ (0x9E3779B97F4A7C15ull * ((uint64_t)cdp + 1)));
+  k = mix64(k ^ (((uint64_t)t0 << 32) | gen));
+  k = mix64(k ^ (((uint64_t)member << 32) | draw));
+  return (uint32_t)(k >> 32);
+}
+
+__device__ __forceinline__ float de_unit(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// Index among n - nex candidates, mapped past the excluded members.
+__device__ __forceinline__ int de_pick(uint32_t h, int n, int e0, int e1, int e2, int nex) {
+  int v = (int)(h % (uint32_t)(n - nex));
+  // sort the exclusions (nex <= 3)
+  int x0 = e0, x1 = e1, x2 = e2;
+  if (nex >= 2 && x1 < x0) {
+    int t = x0;
+    x0 = x1;
+    x1 = t;
+  }
+  if (nex == 3) {
+    if (x2 < x1) {
+      int t = x1;
+      x1 = x2;
+      x2 = t;
+    }
+    if (x1 < x0) {
+      int t = x0;
+      x0 = x1;
+      x1 = t;
+    }
+  }
+  if (v >= x0) v++;
+  if (nex >= 2 && v >= x1) v++;
+  if (nex == 3 && v >= x2) v++;
+  return v;
+}
+
+// Fitness in the oracle's FP32 operation order (no contraction).
+template <class LD>
+__device__ void de_eval_exact(const SearchArgs& a, const LD& ld, int k0, int k1, int t0, float pa, float pb, float pc,
+                              float& sem, float& stk) {
+#pragma clang fp contract(off)
+  const int w = a.w;
+  float num[SB_MAX_W];
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.f;
+  float den = 0.f, lin = 0.f, mm = 0.f;
+  int err = 0;
+  const float t0s = a.dt_f * (float)t0;
+  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  for (int k = k0; k < k1; k++) {
+    const SlotF s = a.slots_p[k];
+    float t = t0s + pa * s.m;
+    float tsq = t * t;
+    float bm = pb * s.m2;
+    float ch = pc * s.h;
+    t = tsq + bm;
+    t = t + ch;
+    t = t < 0.0f ? -1.0f : sqrtf(t);
+    const float ti = t * a.idt_f;
+    if (ti >= lo && ti < hi) {
+      const int it = (int)ti;
+      const float x = ti - (float)it;
+      const int e = it - a.tau;
+      float s1 = ld(s.row, e), s0;
+#pragma unroll
+      for (int j = 0; j < SB_MAX_W; j++) {
+        if (j < w) {
+          s0 = s1;
+          s1 = ld(s.row, e + j + 1);
+          float d = s1 - s0;
+          float v = d * x + s0;
+          num[j] += v;
+          float vv = v * v;
+          den += vv;
+          lin += v;
+        }
+      }
+      mm += 1.0f;
+    } else {
+      err++;
+    }
+  }
+  float acsq = 0.f;
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++)
+    if (j < w) {
+      float q = num[j] * num[j];
+      acsq += q;
+    }
+  if ((double)den > 1e-13 && (double)mm > 1e-13 && err < 2) {
+    sem = acsq / (den * mm);
+    stk = lin / ((float)w * mm);
+  } else {
+    sem = 0.f;
+    stk = 0.f;
+  }
+}
+
+// Fast fitness: slot geometry scaled to samples, contracted FMAs, hardware
+// sqrt, window loads from a raw buffer (out-of-range lanes read zeros).
+template <int W, int STORE>
+__device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, float pa, float pb, float pc, float& sem,
+                             float& stk) {
+  constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
+  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const int nrec = (int)(a.stride * ESZ);
+  const char* __restrict__ base = (const char*)a.samples;
+  float num[W], den = 0.f;
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < W; j++) num[j] = 0.f;
+  SlotF nx = a.slots[k0 < k1 ? k0 : 0];
+  for (int k = k0; k < k1; k++) {
+    const SlotF s = nx;
+    nx = a.slots[k + 1 < k1 ? k + 1 : k];
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
+    const float y = fmaf(pa, s.m, T0);
+    const float tt = fmaf(y, y, fmaf(pb, s.m2, pc * s.h));
+    const float ts = __builtin_amdgcn_sqrtf(tt);
+    const bool ok = (ts >= lo) && (ts < hi);
+    const int e = (int)ts - a.tau;
+    const float x = ok ? __builtin_amdgcn_fractf(ts) : 0.f;
+    unsigned off, sh;
+    if constexpr (STORE == SB_STORE_F32) {
+      off = ok ? (unsigned)e * 4u : kOob;
+      sh = 0;
+    } else {
+      off = ok ? ((unsigned)e & ~1u) * 2u : kOob;
+      sh = ((unsigned)e & 1u) << 4;
+    }
+    const Win<W + 1> sv = load_win<W + 1, STORE>(rs, off, sh);
+    cnt += ok ? 1 : 0;
+    if constexpr (STORE == SB_STORE_F32) {
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const float v = fmaf(sv.v[j + 1] - sv.v[j], x, sv.v[j]);
+        num[j] += v;
+        den = fmaf(v, v, den);
+      }
+    } else {
+      float q = 0.f;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const float v = fmaf(sv.v[j + 1] - sv.v[j], x, sv.v[j]);
+        num[j] = fmaf(s.scale, v, num[j]);
+        q = fmaf(v, v, q);
+      }
+      den = fmaf(s.scale2, q, den);
+    }
+  }
+  float acsq = 0.f, lin = 0.f;
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    acsq = fmaf(num[j], num[j], acsq);
+    lin += num[j];
+  }
+  const int S = k1 - k0;
+  const float m = (float)cnt;
+  if (den > 1e-13f && cnt > 0 && S - cnt < 2) {
+    sem = acsq * __builtin_amdgcn_rcpf(den * m);
+    stk = lin * __builtin_amdgcn_rcpf((float)W * m);
+  } else {
+    sem = 0.f;
+    stk = 0.f;
+  }
+}
+
+template <int W, bool EXACT, int STORE, class LD>
+__global__ __launch_bounds__(256) void k_crs_de(SearchArgs a, DeArgs d, LD ld) {
+#pragma clang fp contract(off)
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;
+  const int cdp = wi.cdp;
+  const int P2 = d.pop <= 4 ? 4 : d.pop <= 8 ? 8 : d.pop <= 16 ? 16 : d.pop <= 32 ? 32 : 64;
+  const int per_block = 256 / P2;
+  const int lane = threadIdx.x & 63;
+  const int seg = (int)threadIdx.x / P2;      // t0 slot within the block
+  const int member = (int)threadIdx.x % P2;
+  const int seg_base = (lane / P2) * P2;       // first lane of my segment in the wave
+  const int t0 = wi.tile * per_block + seg;
+  const bool live = member < d.pop && t0 < a.ns;
+  const int t0c = t0 < a.ns ? t0 : a.ns - 1;
+  const int k0 = a.slot_off[cdp], k1 = a.slot_off[cdp + 1];
+
+  auto eval = [&](float pa, float pb, float pc, float& sem, float& stk) {
+    if constexpr (EXACT)
+      de_eval_exact(a, ld, k0, k1, t0c, pa, pb, pc, sem, stk);
+    else
+      de_eval_fast<W, STORE>(a, k0, k1, (float)t0c, pa, pb, pc, sem, stk);
+  };
+
+  float x[3], fit = 0.f, stk = 0.f;
+  const int mi = member < d.pop ? member : 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float u = de_unit(de_hash(d.seed, cdp, t0c, 0, mi, k));
+    const float span = d.hi[k] - d.lo[k];
+    const float us = u * span;
+    x[k] = d.lo[k] + us;
+  }
+  eval(x[0], x[1], x[2], fit, stk);
+
+  for (int gen = 1; gen <= d.ngen; gen++) {
+    const int r1 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 0), d.pop, mi, 0, 0, 1);
+    const int r2 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 1), d.pop, mi, r1, 0, 2);
+    const int r3 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 2), d.pop, mi, r1, r2, 3);
+    const int jr = (int)(de_hash(d.seed, cdp, t0c, gen, mi, 3) % 3u);
+    float tr[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      // every lane of the segment takes part in the shuffles
+      const float x1 = __shfl(x[k], seg_base + r1, 64);
+      const float x2 = __shfl(x[k], seg_base + r2, 64);
+      const float x3 = __shfl(x[k], seg_base + r3, 64);
+      const uint32_t u = de_hash(d.seed, cdp, t0c, gen, mi, 4 + k) >> 8;
+      if (u < d.cr_thr || k == jr) {
+        float diff = x2 - x3;
+        float fd = d.F * diff;
+        float v = x1 + fd;
+        tr[k] = fminf(fmaxf(v, d.lo[k]), d.hi[k]);
+      } else {
+        tr[k] = x[k];
+      }
+    }
+    float tf, ts;
+    eval(tr[0], tr[1], tr[2], tf, ts);
+    if (tf >= fit) {
+      x[0] = tr[0];
+      x[1] = tr[1];
+      x[2] = tr[2];
+      fit = tf;
+      stk = ts;
+    }
+  }
+
+  // Best member of the segment: largest fitness, lowest index on ties.
+  float bf = live ? fit : -1.f;
+  int bi = member;
+  for (int o = 1; o < P2; o <<= 1) {
+    const float of = __shfl_xor(bf, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (of > bf || (of == bf && oi < bi)) {
+      bf = of;
+      bi = oi;
+    }
+  }
+  const float ba = __shfl(x[0], seg_base + bi, 64);
+  const float bb = __shfl(x[1], seg_base + bi, 64);
+  const float bc = __shfl(x[2], seg_base + bi, 64);
+  const float bs = __shfl(stk, seg_base + bi, 64);
+  if (member == 0 && t0 < a.ns) {
+    const int64_t o = (int64_t)cdp * a.ns + t0;
+    a.out_index[o] = -1;
+    a.out_a[o] = ba;
+    a.out_b[o] = bb;
+    a.out_c[o] = (float)((double)bc / 1e6);
+    a.out_coher[o] = bf;
+    a.out_stack[o] = bs;
+  }
+}
+
+bool de_supported(int w, int pop) { return pop >= 4 && pop <= 64 && w >= 1 && w <= SB_MAX_W; }
+
+template <bool EXACT, int STORE, class LD>
+static hipError_t de_w(const SearchArgs& a, const DeArgs& d, const LD& ld, int nblocks, hipStream_t st) {
+  if constexpr (EXACT) {
+    hipLaunchKernelGGL((k_crs_de<1, true, STORE, LD>), dim3(nblocks), dim3(256), 0, st, a, d, ld);
+    return hipGetLastError();
+  } else {
+    switch (a.w) {
+      case 1: hipLaunchKernelGGL((k_crs_de<1, false, STORE, LD>), dim3(nblocks), dim3(256), 0, st, a, d, ld); break;
+      case 3: hipLaunchKernelGGL((k_crs_de<3, false, STORE, LD>), dim3(nblocks), dim3(256), 0, st, a, d, ld); break;
+      case 5: hipLaunchKernelGGL((k_crs_de<5, false, STORE, LD>), dim3(nblocks), dim3(256), 0, st, a, d, ld); break;
+      case 7: hipLaunchKernelGGL((k_crs_de<7, false, STORE, LD>), dim3(nblocks), dim3(256), 0, st, a, d, ld); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+}
+
+hipError_t launch_de(const SearchArgs& a, const DeArgs& d, int store, int exact, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  if (store == SB_STORE_F32) {
+    LoadF32 ld{(const float*)a.samples, a.stride};
+    return exact ? de_w<true, SB_STORE_F32>(a, d, ld, nblocks, st) : de_w<false, SB_STORE_F32>(a, d, ld, nblocks, st);
+  }
+  if (store == SB_STORE_F16) {
+    LoadF16 ld{(const __half*)a.samples, a.scale, a.stride};
+    return exact ? de_w<true, SB_STORE_F16>(a, d, ld, nblocks, st) : de_w<false, SB_STORE_F16>(a, d, ld, nblocks, st);
+  }
+  LoadBF16 ld{(const uint16_t*)a.samples, a.scale, a.stride};
+  return exact ? de_w<true, SB_STORE_BF16>(a, d, ld, nblocks, st) : de_w<false, SB_STORE_BF16>(a, d, ld, nblocks, st);
+}
+
+}  // namespace sb

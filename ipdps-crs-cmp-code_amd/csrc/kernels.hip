@@ -1,0 +1,704 @@
+// gfx950 kernels of the CMP / CRS semblance search.
+//
+// Layout in HBM: samples [ntr][stride] (trace-major, CDP-binned), one slot
+// table per method (CMP: each CDP's own traces; CRS: each CDP's supergather of
+// neighbour-CDP traces, CRS/OpenMP/src/main.cpp:179-204) holding the trace row
+// and its scaled geometry, and per-(CDP, t0) result arrays.
+//
+// Hot kernel (k_search_fast): lanes of a wave are 64 consecutive t0 of one
+// CDP; every lane walks the CDP's traces in order for a small chunk of
+// parameters, so a wave's 64 sample-window loads for one trace fall on a
+// few consecutive cache lines (moveout is smooth in t0) and the trace
+// geometry and parameter products are wave-uniform.  Windowed sums stay in
+// registers; the argmax over parameters is fused (64-bit packed key,
+// atomicMax across parameter chunks), so the reference's num/stt arrays
+// (CMP/OpenMP/src/main.cpp:93-94) are never materialised.  The winner of
+// every (CDP, t0) is re-evaluated by k_finalize in the reference's precision
+// and operation order.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include "device_common.hpp"
+#include "device_types.hpp"
+#include "launch.hpp"
+#include "semblance.h"
+
+namespace sb {
+
+// ------------------------------------------------------ per-trace geometry
+// CMP reference arithmetic (float, CMP/OpenMP/src/main.cpp:126-136) and the
+// FP64 geometry (the CRS reference arithmetic, CRS/OpenMP/src/main.cpp:164-177).
+__global__ void k_prep_traces(const TraceGeom* __restrict__ g, int ntr, float* __restrict__ h_ref32,
+                              double* __restrict__ h0, double* __restrict__ m0x, double* __restrict__ m0y) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntr) return;
+  const TraceGeom t = g[i];
+  {
+    float s = (float)t.scalco;
+    if (-1e-13 < (double)s && (double)s < 1e-13)
+      s = 1.0f;
+    else if (s < 0)
+      s = 1.0f / s;
+    float hx = ((float)t.gx - (float)t.sx) * s;
+    float hy = ((float)t.gy - (float)t.sy) * s;
+    float hxx = hx * hx;
+    float hyy = hy * hy;
+    float hh = hxx + hyy;
+    h_ref32[i] = (float)(0.25 * (double)hh / 1e6);
+  }
+  {
+    double s = (double)t.scalco;
+    if (-1e-13 < s && s < 1e-13)
+      s = 1.0;
+    else if (s < 0)
+      s = 1.0 / s;
+    m0x[i] = ((double)t.gx + (double)t.sx) * s * 0.5;
+    m0y[i] = ((double)t.gy + (double)t.sy) * s * 0.5;
+    double hx = ((double)t.gx - (double)t.sx) * s;
+    double hy = ((double)t.gy - (double)t.sy) * s;
+    double hxx = hx * hx;
+    double hyy = hy * hy;
+    h0[i] = 0.25 * (hxx + hyy) / 1e6;
+  }
+}
+
+// CMP slot table: slot k of CDP i is row cdp_off[i] + k, H = h * idt^2.
+__global__ void k_cmp_slots(int ntr, const double* __restrict__ h0, const float* __restrict__ scale, double idt,
+                            SlotF* __restrict__ slots) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntr) return;
+  SlotF s;
+  s.row = r;
+  s.m = 0.f;
+  s.m2 = 0.f;
+  s.h = (float)(h0[r] * idt * idt);
+  s.scale = scale ? scale[r] : 1.f;
+  s.scale2 = s.scale * s.scale;
+  s.pad0 = s.pad1 = 0;
+  slots[r] = s;
+}
+
+// CRS supergather slot tables, one block per CDP: neighbour CDPs in list
+// order, traces in gather order; distances to the CDP's first trace
+// (CRS/OpenMP/src/main.cpp:179-198, 287-311).
+__global__ void k_crs_slots(const int32_t* __restrict__ cdp_off, const int32_t* __restrict__ nbr_off,
+                            const int32_t* __restrict__ nbr, const int32_t* __restrict__ slot_off,
+                            const double* __restrict__ m0x, const double* __restrict__ m0y,
+                            const double* __restrict__ h0, const float* __restrict__ scale, double idt,
+                            SlotF* __restrict__ sf, SlotD* __restrict__ sd, SlotF* __restrict__ sp) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x;
+  const int ref = cdp_off[i];
+  const double rx = m0x[ref], ry = m0y[ref];
+  int base = slot_off[i];
+  for (int q = nbr_off[i]; q < nbr_off[i + 1]; q++) {
+    const int j = nbr[q];
+    const int r0 = cdp_off[j], r1 = cdp_off[j + 1];
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+      double dx = m0x[r] - rx;
+      double dy = m0y[r] - ry;
+      double dxx = dx * dx;
+      double dyy = dy * dy;
+      double m2 = dxx + dyy;
+      double m = sqrt(m2);
+      double h = h0[r];
+      SlotD d;
+      d.m = m;
+      d.m2 = m2;
+      d.h = h;
+      d.row = r;
+      d.pad = 0;
+      sd[base + (r - r0)] = d;
+      SlotF f;
+      f.row = r;
+      f.m = (float)(m * idt);
+      f.m2 = (float)(m2 * idt * idt);
+      f.h = (float)(h * idt * idt);
+      f.scale = scale ? scale[r] : 1.f;
+      f.scale2 = f.scale * f.scale;
+      f.pad0 = f.pad1 = 0;
+      sf[base + (r - r0)] = f;
+      f.m = (float)m;
+      f.m2 = (float)m2;
+      f.h = (float)h;
+      sp[base + (r - r0)] = f;
+    }
+    base += r1 - r0;
+  }
+}
+
+// ------------------------------------------------------------ quantisation
+// F16/BF16 store: per-trace power-of-two scale >= max|s|, so s/scale is exact
+// before the final rounding and restoring is an exact FP32 multiply.
+template <int STORE>
+__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ src, int ns, int64_t stride,
+                                                  void* __restrict__ dst, float* __restrict__ scale) {
+  const int64_t row = blockIdx.x;
+  const float* s = src + row * (int64_t)ns;
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) mx = fmaxf(mx, fabsf(s[i]));
+  __shared__ float red[256];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  mx = red[0];
+  float sc = 1.f;
+  if (mx > 0.f) {
+    int e;
+    frexpf(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
+    sc = ldexpf(1.f, e);
+  }
+  const float inv = 1.f / sc;
+  if (threadIdx.x == 0) scale[row] = sc;
+  for (int i = threadIdx.x; i < (int)stride; i += blockDim.x) {
+    float v = i < ns ? s[i] * inv : 0.f;
+    if (STORE == SB_STORE_F16) {
+      ((__half*)dst)[row * stride + i] = __float2half_rn(v);
+    } else {
+      ((__hip_bfloat16*)dst)[row * stride + i] = __float2bfloat16(v);
+    }
+  }
+}
+
+// Plain FP32 copy into the padded row layout.
+__global__ __launch_bounds__(256) void k_pad_rows(const float* __restrict__ src, int ns, int64_t stride,
+                                                  float* __restrict__ dst) {
+  const int64_t row = blockIdx.x;
+  for (int i = threadIdx.x; i < (int)stride; i += blockDim.x)
+    dst[row * stride + i] = i < ns ? src[row * (int64_t)ns + i] : 0.f;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------- reference-order evaluation
+// One (CDP, t0, c) of CMP exactly as CMP/OpenMP/src/main.cpp:150-203: float,
+// no contraction, correctly rounded sqrt/div, traces in gather order.
+template <class LD>
+__device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int cid, float& sem,
+                               float& stk) {
+#pragma clang fp contract(off)
+  const float c = a.cf0 + a.cfinc * (float)cid;
+  float tt0 = a.dt_f * (float)t0;
+  tt0 = tt0 * tt0;
+  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const int w = a.w;
+  float num[SB_MAX_W];
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.f;
+  float den = 0.f, lin = 0.f, m = 0.f;
+  int err = 0;
+  const int r0 = a.cdp_off[cdp], r1 = a.cdp_off[cdp + 1];
+  for (int row = r0; row < r1; row++) {
+    float ch = c * a.h_ref32[row];
+    float t = sqrtf(tt0 + ch) * a.idt_f;
+    if (t >= lo && t < hi) {
+      const int it = (int)t;
+      const float x = t - (float)it;
+      const int e = it - a.tau;
+      float s1 = ld(row, e), s0;
+#pragma unroll
+      for (int j = 0; j < SB_MAX_W; j++) {
+        if (j < w) {
+          s0 = s1;
+          s1 = ld(row, e + j + 1);
+          float d = s1 - s0;
+          float v = d * x + s0;
+          num[j] += v;
+          float vv = v * v;
+          den += vv;
+          lin += v;
+        }
+      }
+      m += 1.0f;
+    } else {
+      err++;
+    }
+  }
+  float acsq = 0.f;
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++)
+    if (j < w) {
+      float q = num[j] * num[j];
+      acsq += q;
+    }
+  const float wf = (float)w;
+  if ((double)den > 1e-13 && (double)m > 1e-13 && (double)wf > 1e-13 && err < 2) {
+    sem = acsq / (den * m);
+    stk = lin / (wf * m);
+  } else {
+    sem = -1.0f;
+    stk = -1.0f;
+  }
+}
+
+// One (CDP, t0, par) of CRS exactly as CRS/OpenMP/src/main.cpp:210-265: double.
+template <class LD>
+__device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int p, double& sem,
+                               double& stk) {
+#pragma clang fp contract(off)
+  const int nbnc = a.nc * a.nb;
+  const double pa = a.da0 + a.dinc_a * (double)(p / nbnc);
+  const double pb = a.db0 + a.dinc_b * (double)((p / a.nc) % a.nb);
+  const double pc = a.dc0 + a.dinc_c * (double)(p % a.nc);
+  const double tt0 = a.dt_d * (double)t0;
+  const double lo = (double)a.tau, hi = (double)(a.ns - a.tau - 1);
+  const int w = a.w;
+  double num[SB_MAX_W];
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.0;
+  double den = 0.0, lin = 0.0, mm = 0.0;
+  int err = 0;
+  const int k0 = a.slot_off[cdp], k1 = a.slot_off[cdp + 1];
+  for (int k = k0; k < k1; k++) {
+    const SlotD s = a.slots_d[k];
+    double t = tt0 + pa * s.m;
+    double tsq = t * t;
+    double bm = pb * s.m2;
+    double ch = pc * s.h;
+    t = tsq + bm;
+    t = t + ch;
+    t = t < 0.0 ? -1.0 : sqrt(t);
+    const double ti = t * a.idt_d;
+    if (ti >= lo && ti < hi) {
+      const int it = (int)ti;
+      const double x = ti - (double)it;
+      const int e = it - a.tau;
+      double s1 = (double)ld(s.row, e), s0;
+#pragma unroll
+      for (int j = 0; j < SB_MAX_W; j++) {
+        if (j < w) {
+          s0 = s1;
+          s1 = (double)ld(s.row, e + j + 1);
+          double d = s1 - s0;
+          double v = d * x + s0;
+          num[j] += v;
+          double vv = v * v;
+          den += vv;
+          lin += v;
+        }
+      }
+      mm += 1.0;
+    } else {
+      err++;
+    }
+  }
+  double acsq = 0.0;
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++)
+    if (j < w) {
+      double q = num[j] * num[j];
+      acsq += q;
+    }
+  if (den > 1e-13 && mm > 1e-13 && (double)w > 1e-13 && err < 2) {
+    sem = acsq / (den * mm);
+    stk = lin / ((double)w * mm);
+  } else {
+    sem = 0.0;
+    stk = 0.0;
+  }
+}
+
+// Output encodings (CMP/OpenMP/src/main.cpp:219-221,253; CRS/OpenMP/src/main.cpp:282-284,348-350).
+__device__ void write_cmp(const SearchArgs& a, int64_t o, int idx, float sem, float stk) {
+#pragma clang fp contract(off)
+  a.out_index[o] = idx;
+  a.out_coher[o] = sem;
+  a.out_stack[o] = stk;
+  a.out_a[o] = 0.f;
+  a.out_b[o] = 0.f;
+  a.out_c[o] = idx < 0 ? 0.0f : (float)((double)(a.cf0 + a.cfinc * (float)idx) / 1e6);
+}
+
+__device__ void write_crs(const SearchArgs& a, int64_t o, int idx, double sem, double stk) {
+#pragma clang fp contract(off)
+  const int nbnc = a.nc * a.nb;
+  a.out_index[o] = idx;
+  a.out_coher[o] = (float)sem;
+  a.out_stack[o] = (float)stk;
+  a.out_a[o] = (float)(a.da0 + a.dinc_a * (double)(idx / nbnc));
+  a.out_b[o] = (float)(a.db0 + a.dinc_b * (double)((idx / a.nc) % a.nb));
+  a.out_c[o] = (float)((a.dc0 + a.dinc_c * (double)(idx % a.nc)) / 1e6);
+}
+
+// ------------------------------------------------------ SB_MODE_EXACT search
+// One thread per (CDP, t0) scanning every parameter in order with the
+// reference's strict '>' arg-max (CMP/OpenMP/src/main.cpp:207-222,
+// CRS/OpenMP/src/main.cpp:270-285).  Work items carry (cdp, 64-t0 tile).
+template <class LD>
+__global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;
+  const int t0 = wi.tile * 64 + (int)threadIdx.x;
+  if (t0 >= a.ns) return;
+  const int64_t o = (int64_t)wi.cdp * a.ns + t0;
+  if (a.method == SB_METHOD_CMP) {
+    float max_sem = 0.f, best_stk = 0.f;
+    int max_c = -1;
+    for (int c = 0; c < a.nc; c++) {
+      float sem, stk;
+      eval_cmp_exact(a, ld, wi.cdp, t0, c, sem, stk);
+      if (sem > max_sem) {
+        max_sem = sem;
+        max_c = c;
+        best_stk = stk;
+      }
+    }
+    write_cmp(a, o, max_c, max_sem, max_c > -1 ? best_stk : 0.f);
+  } else {
+    double max_sem = 0.0, best_stk = 0.0;
+    int max_p = -1;
+    for (int p = 0; p < a.npar; p++) {
+      double sem, stk;
+      eval_crs_exact(a, ld, wi.cdp, t0, p, sem, stk);
+      if (sem > max_sem) {
+        max_sem = sem;
+        max_p = p;
+        best_stk = stk;
+      }
+    }
+    if (max_p < 0) {
+      // No parameter beat 0: the reference keeps the absolute index 0, i.e.
+      // (t0 = 0, par 0) -- stack = stt[0] of this CDP (quirk Q7).
+      double sem0;
+      eval_crs_exact(a, ld, wi.cdp, 0, 0, sem0, best_stk);
+      max_p = 0;
+    }
+    write_crs(a, o, max_p, max_sem, best_stk);
+  }
+}
+
+// ------------------------------------------------------------- finalize
+// Decodes the packed best key of every (CDP, t0) and re-evaluates the winner
+// in the reference's precision and order; writes the output arrays.
+template <class LD>
+__global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
+  const int64_t n = (int64_t)a.ncdp * a.ns;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  const int cdp = (int)(o / a.ns), t0 = (int)(o % a.ns);
+  const unsigned long long key = a.keys[o];
+  const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
+  if (a.method == SB_METHOD_CMP) {
+    float sem = 0.f, stk = 0.f;
+    if (p >= 0) eval_cmp_exact(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0 && sem > 0.f)
+      write_cmp(a, o, p, sem, stk);
+    else
+      write_cmp(a, o, -1, 0.f, 0.f);
+  } else {
+    double sem = 0.0, stk = 0.0;
+    if (p >= 0) eval_crs_exact(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0 && sem > 0.0) {
+      write_crs(a, o, p, sem, stk);
+    } else {
+      double s0;
+      eval_crs_exact(a, ld, cdp, 0, 0, s0, stk);
+      write_crs(a, o, 0, 0.0, stk);
+    }
+  }
+}
+
+// ---------------------------------------------------- SB_MODE_FAST search
+template <int W, int R, int CC, bool CRS, int STORE>
+__global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
+  constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
+  __shared__ unsigned long long s_best[4][64 * R];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;
+  const int cdp = wi.cdp;
+  const int k0 = a.slot_off[cdp];
+  const int S = a.slot_off[cdp + 1] - k0;
+  if (S <= 0) return;  // no traces: nothing beats 0, keys stay empty
+  const SlotF* __restrict__ sl = a.slots + k0;
+
+  float T0[R], T0sq[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    T0[r] = (float)(wi.tile * 64 * R + r * 64 + lane);
+    T0sq[r] = T0[r] * T0[r];
+  }
+  // Contiguous parameter sub-range of this wave, a multiple of CC long.
+  const int np = wi.p1 - wi.p0;
+  const int per = ((np + 4 * CC - 1) / (4 * CC)) * CC;
+  const int pw0 = wi.p0 + wave * per;
+  const int pw1 = min(pw0 + per, wi.p1);
+
+  unsigned long long best[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) best[r] = 0ull;
+
+  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const int nbnc = a.nb * a.nc;
+  const int nrec = (int)(a.stride * ESZ);
+  const char* __restrict__ base = (const char*)a.samples;
+
+  for (int p = pw0; p < pw1; p += CC) {
+    float pa[CC], pb[CC], pc[CC];
+    bool act[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      const int q = p + ci;
+      act[ci] = q < pw1;
+      const int qq = act[ci] ? q : p;
+      if constexpr (CRS) {
+        const int ia = qq / nbnc, ib = (qq / a.nc) % a.nb, ic = qq % a.nc;
+        pa[ci] = __fadd_rn(a.fa0, __fmul_rn(a.finc_a, (float)ia));
+        pb[ci] = __fadd_rn(a.fb0, __fmul_rn(a.finc_b, (float)ib));
+        pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)ic));
+      } else {
+        pa[ci] = 0.f;
+        pb[ci] = 0.f;
+        pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)qq));
+      }
+    }
+    float num[R][CC][W], den[R][CC];
+    int cnt[R][CC];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        den[r][ci] = 0.f;
+        cnt[r][ci] = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) num[r][ci][j] = 0.f;
+      }
+
+    SlotF nx = sl[0];
+    for (int k = 0; k < S; k++) {
+      const SlotF s = nx;
+      nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
+      const __amdgpu_buffer_rsrc_t rs =
+          __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
+      float U[CC], V[CC];
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        if constexpr (CRS) {
+          V[ci] = pa[ci] * s.m;
+          U[ci] = fmaf(pb[ci], s.m2, pc[ci] * s.h);
+        } else {
+          V[ci] = 0.f;
+          U[ci] = pc[ci] * s.h;
+        }
+      }
+      unsigned off[R][CC], shf[R][CC];
+      float xx[R][CC];
+      bool okk[R][CC];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) {
+          float tt;
+          if constexpr (CRS) {
+            const float y = T0[r] + V[ci];
+            tt = fmaf(y, y, U[ci]);
+          } else {
+            tt = T0sq[r] + U[ci];
+          }
+          const float ts = __builtin_amdgcn_sqrtf(tt);
+          const bool ok = (ts >= lo) && (ts < hi);  // false for NaN (tt < 0)
+          const int e = (int)ts - a.tau;
+          okk[r][ci] = ok;
+          xx[r][ci] = ok ? __builtin_amdgcn_fractf(ts) : 0.f;
+          if constexpr (STORE == SB_STORE_F32) {
+            off[r][ci] = ok ? (unsigned)e * 4u : kOob;
+            shf[r][ci] = 0;
+          } else {
+            off[r][ci] = ok ? ((unsigned)e & ~1u) * 2u : kOob;
+            shf[r][ci] = ((unsigned)e & 1u) << 4;
+          }
+        }
+      }
+      Win<W + 1> sv[R][CC];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) sv[r][ci] = load_win<W + 1, STORE>(rs, off[r][ci], shf[r][ci]);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) {
+          cnt[r][ci] += okk[r][ci] ? 1 : 0;
+          const float x = xx[r][ci];
+          if constexpr (STORE == SB_STORE_F32) {
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+              const float v = fmaf(sv[r][ci].v[j + 1] - sv[r][ci].v[j], x, sv[r][ci].v[j]);
+              num[r][ci][j] += v;
+              den[r][ci] = fmaf(v, v, den[r][ci]);
+            }
+          } else {
+            float q = 0.f;
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+              const float v = fmaf(sv[r][ci].v[j + 1] - sv[r][ci].v[j], x, sv[r][ci].v[j]);
+              num[r][ci][j] = fmaf(s.scale, v, num[r][ci][j]);
+              q = fmaf(v, v, q);
+            }
+            den[r][ci] = fmaf(s.scale2, q, den[r][ci]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        float acsq = 0.f;
+#pragma unroll
+        for (int j = 0; j < W; j++) acsq = fmaf(num[r][ci][j], num[r][ci][j], acsq);
+        const float m = (float)cnt[r][ci];
+        const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && (S - cnt[r][ci]) < 2;
+        const float sem = acsq * __builtin_amdgcn_rcpf(den[r][ci] * m);
+        if (valid && sem > 0.f) {
+          const unsigned long long key =
+              ((unsigned long long)__float_as_uint(sem) << 32) | (unsigned long long)(~(uint32_t)(p + ci));
+          best[r] = key > best[r] ? key : best[r];
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; r++) s_best[wave][r * 64 + lane] = best[r];
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      unsigned long long m = s_best[0][r * 64 + lane];
+#pragma unroll
+      for (int q = 1; q < 4; q++) {
+        const unsigned long long v = s_best[q][r * 64 + lane];
+        m = v > m ? v : m;
+      }
+      const int t0 = wi.tile * 64 * R + r * 64 + lane;
+      if (t0 < a.ns && m) atomicMax(&a.keys[(int64_t)cdp * a.ns + t0], m);
+    }
+  }
+}
+
+// ------------------------------------------------------------- launchers
+#define SB_CHECK_LAUNCH() (hipGetLastError())
+
+hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
+                              hipStream_t st) {
+  if (ntr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prep_traces, dim3((ntr + 255) / 256), dim3(256), 0, st, g, ntr, h_ref32, h0, m0x, m0y);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_cmp_slots(int ntr, const double* h0, const float* scale, double idt, SlotF* slots,
+                            hipStream_t st) {
+  if (ntr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cmp_slots, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, h0, scale, idt, slots);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_crs_slots(int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
+                            const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
+                            const float* scale, double idt, SlotF* sf, SlotD* sd, SlotF* sp, hipStream_t st) {
+  if (ncdp <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_crs_slots, dim3(ncdp), dim3(256), 0, st, cdp_off, nbr_off, nbr, slot_off, m0x, m0y, h0, scale,
+                     idt, sf, sd, sp);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int store, void* dst, float* scale,
+                        hipStream_t st) {
+  if (ntr <= 0) return hipSuccess;
+  if (store == SB_STORE_F32)
+    hipLaunchKernelGGL(k_pad_rows, dim3(ntr), dim3(256), 0, st, src, ns, stride, (float*)dst);
+  else if (store == SB_STORE_F16)
+    hipLaunchKernelGGL(k_quantize<SB_STORE_F16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
+  else
+    hipLaunchKernelGGL(k_quantize<SB_STORE_BF16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return SB_CHECK_LAUNCH();
+}
+
+template <class LD>
+static hipError_t finalize_with(const SearchArgs& a, const LD& ld, hipStream_t st) {
+  const int64_t n = (int64_t)a.ncdp * a.ns;
+  hipLaunchKernelGGL(k_finalize<LD>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
+  return SB_CHECK_LAUNCH();
+}
+
+template <class LD>
+static hipError_t exact_with(const SearchArgs& a, const LD& ld, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_search_exact<LD>, dim3(nblocks), dim3(64), 0, st, a, ld);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st) {
+  if ((int64_t)a.ncdp * a.ns <= 0) return hipSuccess;
+  if (store == SB_STORE_F32) return finalize_with(a, LoadF32{(const float*)a.samples, a.stride}, st);
+  if (store == SB_STORE_F16) return finalize_with(a, LoadF16{(const __half*)a.samples, a.scale, a.stride}, st);
+  return finalize_with(a, LoadBF16{(const uint16_t*)a.samples, a.scale, a.stride}, st);
+}
+
+hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  if (store == SB_STORE_F32) return exact_with(a, LoadF32{(const float*)a.samples, a.stride}, nblocks, st);
+  if (store == SB_STORE_F16)
+    return exact_with(a, LoadF16{(const __half*)a.samples, a.scale, a.stride}, nblocks, st);
+  return exact_with(a, LoadBF16{(const uint16_t*)a.samples, a.scale, a.stride}, nblocks, st);
+}
+
+template <int W, int R, int CC, bool CRS>
+static hipError_t fast_store(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
+  if (store == SB_STORE_F32)
+    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_F32>), dim3(nblocks), dim3(256), 0, st, a);
+  else if (store == SB_STORE_F16)
+    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_F16>), dim3(nblocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_BF16>), dim3(nblocks), dim3(256), 0, st, a);
+  return SB_CHECK_LAUNCH();
+}
+
+template <int R, int CC, bool CRS>
+static hipError_t fast_w(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
+  switch (a.w) {
+    case 1: return fast_store<1, R, CC, CRS>(a, store, nblocks, st);
+    case 3: return fast_store<3, R, CC, CRS>(a, store, nblocks, st);
+    case 5: return fast_store<5, R, CC, CRS>(a, store, nblocks, st);
+    case 7: return fast_store<7, R, CC, CRS>(a, store, nblocks, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool fast_supported(int w) { return w == 1 || w == 3 || w == 5 || w == 7; }
+
+const FastVariant kFastVariants[] = {{2, 4}, {1, 4}, {4, 2}, {2, 2}};
+const int kNumFastVariants = sizeof(kFastVariants) / sizeof(kFastVariants[0]);
+
+hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  const bool crs = a.method == SB_METHOD_CRS;
+  switch (variant) {
+    case 0: return crs ? fast_w<2, 4, true>(a, store, nblocks, st) : fast_w<2, 4, false>(a, store, nblocks, st);
+    case 1: return crs ? fast_w<1, 4, true>(a, store, nblocks, st) : fast_w<1, 4, false>(a, store, nblocks, st);
+    case 2: return crs ? fast_w<4, 2, true>(a, store, nblocks, st) : fast_w<4, 2, false>(a, store, nblocks, st);
+    case 3: return crs ? fast_w<2, 2, true>(a, store, nblocks, st) : fast_w<2, 2, false>(a, store, nblocks, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace sb

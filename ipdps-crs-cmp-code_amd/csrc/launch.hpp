@@ -1,0 +1,38 @@
+// Host-callable launchers of the kernels in kernels.hip / de.hip.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "device_types.hpp"
+
+namespace sb {
+
+struct FastVariant {
+  int R;   // t0 rows per lane (block covers 64*R t0)
+  int CC;  // parameters evaluated together per trace pass
+};
+extern const FastVariant kFastVariants[];
+extern const int kNumFastVariants;
+
+bool fast_supported(int w);
+
+hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
+                              hipStream_t st);
+hipError_t launch_cmp_slots(int ntr, const double* h0, const float* scale, double idt, SlotF* slots,
+                            hipStream_t st);
+hipError_t launch_crs_slots(int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
+                            const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
+                            const float* scale, double idt, SlotF* sf, SlotD* sd, SlotF* sp, hipStream_t st);
+hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int store, void* dst, float* scale,
+                        hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
+
+hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, hipStream_t st);
+hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipStream_t st);
+hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);
+
+// CRS-DE (de.hip).  One work item = (cdp, tile of t0).
+bool de_supported(int w, int pop);
+hipError_t launch_de(const SearchArgs& a, const DeArgs& d, int store, int exact, int nblocks, hipStream_t st);
+
+}  // namespace sb

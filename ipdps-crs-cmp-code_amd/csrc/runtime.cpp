@@ -1,0 +1,720 @@
+// Device context, resident gathers, results and the search entry points of
+// libsemblance (C ABI in include/semblance.h).
+//
+// Replaces the reference's per-CDP host loops (CMP/CUDA/src/main.cu:276-296,
+// CRS/CUDA/src/main.cu:420-450): the whole binned gather is resident in HBM
+// and one launch covers every (CDP, t0 tile, parameter chunk).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "launch.hpp"
+#include "runtime.hpp"
+#include "sb_internal.hpp"
+
+using namespace sb;
+
+namespace sb {
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(SB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPR(call)                                   \
+  do {                                               \
+    hipError_t _e = (call);                          \
+    if (_e != hipSuccess) return hip_fail(_e, #call); \
+  } while (0)
+
+int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
+  *p = nullptr;
+  if (bytes == 0) return SB_OK;
+  std::lock_guard<std::mutex> lk(ctx->mem_mu);
+  if (ctx->used + bytes > ctx->budget)
+    return fail(SB_ERR_BUDGET, "HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
+                                   std::to_string(ctx->budget - ctx->used) + " B left");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, "hipMalloc");
+  }
+  ctx->used += bytes;
+  ctx->sizes[*p] = bytes;
+  return SB_OK;
+}
+
+void ctx_free(sb_ctx* ctx, void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(ctx->mem_mu);
+  auto it = ctx->sizes.find(p);
+  if (it != ctx->sizes.end()) {
+    ctx->used -= it->second;
+    ctx->sizes.erase(it);
+  }
+  (void)hipFree(p);
+}
+
+}  // namespace sb
+
+// ------------------------------------------------------------------ status
+extern "C" const char* sb_strerror(int code) {
+  switch (code) {
+    case SB_OK: return "ok";
+    case SB_ERR_INVALID: return "invalid argument";
+    case SB_ERR_IO: return "i/o error";
+    case SB_ERR_FORMAT: return "bad input format";
+    case SB_ERR_HIP: return "HIP runtime error";
+    case SB_ERR_BUDGET: return "HBM budget exceeded";
+    case SB_ERR_NOMEM: return "out of host memory";
+    case SB_ERR_UNSUPPORTED: return "unsupported configuration";
+    case SB_ERR_STATE: return "invalid state";
+    default: return "unknown error";
+  }
+}
+
+extern "C" const char* sb_version(void) { return "libsemblance 0.1 (gfx950)"; }
+
+// ----------------------------------------------------------------- context
+extern "C" int sb_ctx_create(int device, size_t hbm_budget, sb_ctx** out) {
+  if (!out) return fail(SB_ERR_INVALID, "sb_ctx_create: null out");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(SB_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(SB_ERR_INVALID, "sb_ctx_create: bad device index");
+  HIPR(hipSetDevice(device));
+  std::unique_ptr<sb_ctx> c(new sb_ctx);
+  c->device = device;
+  size_t fr = 0, tot = 0;
+  HIPR(hipMemGetInfo(&fr, &tot));
+  c->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
+  HIPR(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& ev : c->ev) HIPR(hipEventCreate(&ev));
+  hipDeviceProp_t prop;
+  HIPR(hipGetDeviceProperties(&prop, device));
+  c->num_cus = prop.multiProcessorCount;
+  *out = c.release();
+  return SB_OK;
+}
+
+extern "C" void sb_ctx_destroy(sb_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx_free(ctx, ctx->work_buf);
+  ctx_free(ctx, ctx->queue_buf);
+  for (auto& ev : ctx->ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int sb_ctx_memory(const sb_ctx* ctx, size_t* used, size_t* budget) {
+  if (!ctx) return fail(SB_ERR_INVALID, "null ctx");
+  if (used) *used = ctx->used;
+  if (budget) *budget = ctx->budget;
+  return SB_OK;
+}
+
+extern "C" int sb_ctx_sync(sb_ctx* ctx) {
+  if (!ctx) return fail(SB_ERR_INVALID, "null ctx");
+  HIPR(hipSetDevice(ctx->device));
+  HIPR(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+// ----------------------------------------------------------------- gathers
+extern "C" void sb_gather_destroy(sb_gather* g) {
+  if (!g) return;
+  sb_ctx* c = g->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)g->d_samples, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
+                  (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
+                  (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p})
+    ctx_free(c, p);
+  delete g;
+}
+
+namespace {
+
+int upload(sb_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return SB_OK;
+  HIPR(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return SB_OK;
+}
+
+}  // namespace
+
+extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
+                                const sb_trace_geom* geom, const float* samples, int store, const int32_t* nbr_off,
+                                const int32_t* nbr, sb_gather** out) {
+  if (!ctx || !out || !cdp_off || ncdp <= 0 || ns <= 0 || dt_us <= 0 || !geom || !samples)
+    return fail(SB_ERR_INVALID, "sb_gather_create: bad arguments");
+  if (store != SB_STORE_F32 && store != SB_STORE_F16 && store != SB_STORE_BF16)
+    return fail(SB_ERR_INVALID, "sb_gather_create: bad store type");
+  *out = nullptr;
+  if (cdp_off[0] != 0) return fail(SB_ERR_INVALID, "cdp_off[0] must be 0");
+  for (int i = 0; i < ncdp; i++)
+    if (cdp_off[i + 1] <= cdp_off[i]) return fail(SB_ERR_INVALID, "every CDP needs at least one trace");
+  const int ntr = cdp_off[ncdp];
+  int64_t nnbr = 0;
+  if (nbr_off) {
+    if (!nbr || nbr_off[0] != 0) return fail(SB_ERR_INVALID, "bad neighbour lists");
+    for (int i = 0; i < ncdp; i++)
+      if (nbr_off[i + 1] <= nbr_off[i]) return fail(SB_ERR_INVALID, "every CDP needs a non-empty neighbour list");
+    nnbr = nbr_off[ncdp];
+    for (int64_t q = 0; q < nnbr; q++)
+      if (nbr[q] < 0 || nbr[q] >= ncdp) return fail(SB_ERR_INVALID, "neighbour index out of range");
+  }
+  HIPR(hipSetDevice(ctx->device));
+
+  std::unique_ptr<sb_gather, void (*)(sb_gather*)> G(new sb_gather, sb_gather_destroy);
+  G->ctx = ctx;
+  G->serial = ++ctx->gather_serial;
+  G->ncdp = ncdp;
+  G->ntr = ntr;
+  G->ns = ns;
+  G->dt_us = dt_us;
+  G->store = store;
+  // >= 8 zero elements of padding after every row: 16-bit window loads may
+  // read one dword past the last sample.
+  G->stride = ((int64_t)ns + 8 + 63) / 64 * 64;
+  G->cdp_off.assign(cdp_off, cdp_off + ncdp + 1);
+  const size_t esz = store == SB_STORE_F32 ? 4 : 2;
+  int rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * G->stride * esz, &G->d_samples))) return rc;
+  if (store != SB_STORE_F32 && (rc = ctx_alloc(ctx, (size_t)ntr * 4, (void**)&G->d_scale))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, (void**)&G->d_cdp_off))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * 4, (void**)&G->d_h_ref32))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_h0))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0x))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0y))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(SlotF), (void**)&G->d_cmp_slots))) return rc;
+  if ((rc = upload(ctx, G->d_cdp_off, cdp_off, (size_t)(ncdp + 1) * 4))) return rc;
+
+  // Samples: staged in chunks of rows, then padded (F32) or quantised (F16/BF16).
+  {
+    const int64_t chunk_rows = std::max<int64_t>(1, (int64_t)(64 << 20) / ((int64_t)ns * 4));
+    const int64_t rows = std::min<int64_t>(chunk_rows, ntr);
+    void* stage = nullptr;
+    if ((rc = ctx_alloc(ctx, (size_t)rows * ns * 4, &stage))) return rc;
+    for (int64_t r0 = 0; r0 < ntr; r0 += rows) {
+      const int64_t nr = std::min<int64_t>(rows, ntr - r0);
+      hipError_t e = hipMemcpyAsync(stage, samples + r0 * ns, (size_t)nr * ns * 4, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess)
+        e = launch_store((const float*)stage, (int)nr, ns, G->stride, store,
+                         (char*)G->d_samples + (size_t)r0 * G->stride * esz, G->d_scale ? G->d_scale + r0 : nullptr,
+                         ctx->stream);
+      if (e != hipSuccess) {
+        ctx_free(ctx, stage);
+        return hip_fail(e, "sample upload");
+      }
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    ctx_free(ctx, stage);
+    if (e != hipSuccess) return hip_fail(e, "sample upload");
+  }
+
+  // Geometry: FP64 (and the CMP float reference arithmetic) on the device.
+  {
+    std::vector<TraceGeom> tg(ntr);
+    for (int i = 0; i < ntr; i++)
+      tg[i] = TraceGeom{geom[i].cdp, geom[i].scalco, geom[i].sx, geom[i].sy, geom[i].gx, geom[i].gy};
+    void* dg = nullptr;
+    if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(TraceGeom), &dg))) return rc;
+    hipError_t e = hipMemcpyAsync(dg, tg.data(), (size_t)ntr * sizeof(TraceGeom), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_prep_traces((const TraceGeom*)dg, ntr, G->d_h_ref32, G->d_h0, G->d_m0x, G->d_m0y, ctx->stream);
+    const double idt = 1.0 / ((double)dt_us / 1e6);
+    if (e == hipSuccess) e = launch_cmp_slots(ntr, G->d_h0, G->d_scale, idt, G->d_cmp_slots, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx_free(ctx, dg);
+    if (e != hipSuccess) return hip_fail(e, "geometry");
+  }
+
+  // CRS supergathers.
+  if (nbr_off) {
+    G->has_crs = true;
+    G->crs_slot_off.resize(ncdp + 1);
+    G->crs_slot_off[0] = 0;
+    for (int i = 0; i < ncdp; i++) {
+      int64_t s = 0;
+      for (int q = nbr_off[i]; q < nbr_off[i + 1]; q++) s += cdp_off[nbr[q] + 1] - cdp_off[nbr[q]];
+      if (G->crs_slot_off[i] + s > INT32_MAX) return fail(SB_ERR_INVALID, "supergathers too large");
+      G->crs_slot_off[i + 1] = G->crs_slot_off[i] + (int32_t)s;
+    }
+    G->ncrs_slots = G->crs_slot_off[ncdp];
+    void *dnoff = nullptr, *dnbr = nullptr;
+    if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, (void**)&G->d_crs_slot_off))) return rc;
+    if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotF), (void**)&G->d_crs_slots))) return rc;
+    if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotD), (void**)&G->d_crs_slots_d))) return rc;
+    if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotF), (void**)&G->d_crs_slots_p))) return rc;
+    if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, &dnoff))) return rc;
+    if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(nnbr, 1) * 4, &dnbr))) {
+      ctx_free(ctx, dnoff);
+      return rc;
+    }
+    hipError_t e = hipMemcpyAsync(G->d_crs_slot_off, G->crs_slot_off.data(), (size_t)(ncdp + 1) * 4,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dnoff, nbr_off, (size_t)(ncdp + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nnbr) e = hipMemcpyAsync(dnbr, nbr, (size_t)nnbr * 4, hipMemcpyHostToDevice, ctx->stream);
+    const double idt = 1.0 / ((double)dt_us / 1e6);
+    if (e == hipSuccess)
+      e = launch_crs_slots(ncdp, G->d_cdp_off, (const int32_t*)dnoff, (const int32_t*)dnbr, G->d_crs_slot_off,
+                           G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_crs_slots, G->d_crs_slots_d,
+                           G->d_crs_slots_p, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx_free(ctx, dnoff);
+    ctx_free(ctx, dnbr);
+    if (e != hipSuccess) return hip_fail(e, "supergather tables");
+  }
+  G->bytes = 0;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mem_mu);
+    for (void* p : {(void*)G->d_samples, (void*)G->d_scale, (void*)G->d_cdp_off, (void*)G->d_h_ref32, (void*)G->d_h0,
+                    (void*)G->d_m0x, (void*)G->d_m0y, (void*)G->d_cmp_slots, (void*)G->d_crs_slot_off,
+                    (void*)G->d_crs_slots, (void*)G->d_crs_slots_d, (void*)G->d_crs_slots_p}) {
+      auto it = ctx->sizes.find(p);
+      if (it != ctx->sizes.end()) G->bytes += it->second;
+    }
+  }
+  *out = G.release();
+  return SB_OK;
+}
+
+extern "C" int sb_gather_info(const sb_gather* g, int* ncdp, int* ntr, int* ns, int* store, size_t* bytes) {
+  if (!g) return fail(SB_ERR_INVALID, "null gather");
+  if (ncdp) *ncdp = g->ncdp;
+  if (ntr) *ntr = g->ntr;
+  if (ns) *ns = g->ns;
+  if (store) *store = g->store;
+  if (bytes) *bytes = g->bytes;
+  return SB_OK;
+}
+
+// ----------------------------------------------------------------- results
+extern "C" int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out) {
+  if (!ctx || !out || ncdp <= 0 || ns <= 0) return fail(SB_ERR_INVALID, "sb_result_create: bad arguments");
+  *out = nullptr;
+  HIPR(hipSetDevice(ctx->device));
+  std::unique_ptr<sb_result> r(new sb_result);
+  r->ctx = ctx;
+  r->ncdp = ncdp;
+  r->ns = ns;
+  const size_t n = (size_t)ncdp * ns;
+  int rc;
+  void* p = nullptr;
+  if ((rc = ctx_alloc(ctx, n * (6 * 4 + 8), &p))) return rc;
+  r->base = p;
+  r->index = (int32_t*)p;
+  r->a = (float*)((char*)p + n * 4);
+  r->b = (float*)((char*)p + n * 8);
+  r->c = (float*)((char*)p + n * 12);
+  r->coher = (float*)((char*)p + n * 16);
+  r->stack = (float*)((char*)p + n * 20);
+  r->keys = (unsigned long long*)((char*)p + n * 24);
+  *out = r.release();
+  return SB_OK;
+}
+
+extern "C" void sb_result_destroy(sb_result* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->ctx->device);
+  (void)hipStreamSynchronize(r->ctx->stream);
+  ctx_free(r->ctx, r->base);
+  delete r;
+}
+
+extern "C" int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_result* out) {
+  if (!ctx || !r || !out) return fail(SB_ERR_INVALID, "sb_result_download: bad arguments");
+  HIPR(hipSetDevice(ctx->device));
+  const size_t n = (size_t)r->ncdp * r->ns;
+  struct {
+    void* h;
+    const void* d;
+  } m[] = {{out->index, r->index}, {out->a, r->a},         {out->b, r->b},
+           {out->c, r->c},         {out->coher, r->coher}, {out->stack, r->stack}};
+  for (auto& x : m)
+    if (x.h) HIPR(hipMemcpyAsync(x.h, x.d, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPR(hipStreamSynchronize(ctx->stream));
+  return SB_OK;
+}
+
+// --------------------------------------------------------------- searches
+namespace sb {
+
+// Work table: CDPs in descending cost, dealt round-robin over the 8 XCDs
+// (consecutive workgroups land on consecutive XCDs), so every block of a
+// CDP shares one L2 and heavy CDPs start first on every XCD.
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int ntiles, int npar, int pchunk, int* nblocks,
+               const std::string& key) {
+  if (!key.empty() && key == ctx->work_key) {
+    *nblocks = ctx->work_n;
+    return SB_OK;
+  }
+  ctx->work_key.clear();
+  const int ncdp = (int)cost.size();
+  std::vector<int> order(ncdp);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
+  constexpr int kXcd = 8;
+  std::vector<std::vector<WorkItem>> lists(kXcd);
+  std::vector<int64_t> load(kXcd, 0);
+  for (int q = 0; q < ncdp; q++) {
+    const int cdp = order[q];
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost[cdp];
+    for (int t = 0; t < ntiles; t++)
+      for (int c = 0; c < nchunks; c++)
+        lists[x].push_back(WorkItem{cdp, t, c * pchunk, std::min(npar, (c + 1) * pchunk)});
+  }
+  size_t longest = 0;
+  for (auto& l : lists) longest = std::max(longest, l.size());
+  std::vector<WorkItem> items(longest * kXcd, WorkItem{-1, 0, 0, 0});
+  for (int x = 0; x < kXcd; x++)
+    for (size_t j = 0; j < lists[x].size(); j++) items[j * kXcd + x] = lists[x][j];
+  // Trim trailing no-ops.
+  while (!items.empty() && items.back().cdp < 0) items.pop_back();
+  const size_t bytes = std::max<size_t>(items.size(), 1) * sizeof(WorkItem);
+  if (bytes > ctx->work_cap) {
+    ctx_free(ctx, ctx->work_buf);
+    ctx->work_buf = nullptr;
+    ctx->work_cap = 0;
+    int rc = ctx_alloc(ctx, bytes * 2, &ctx->work_buf);
+    if (rc) return rc;
+    ctx->work_cap = bytes * 2;
+  }
+  if (!items.empty())
+    HIPR(hipMemcpyAsync(ctx->work_buf, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                        ctx->stream));
+  // The copy reads a host vector that dies here: wait for it.
+  HIPR(hipStreamSynchronize(ctx->stream));
+  *nblocks = (int)items.size();
+  ctx->work_key = key;
+  ctx->work_n = *nblocks;
+  return SB_OK;
+}
+
+}  // namespace sb
+
+namespace {
+
+// Identifies a work table: gather object + its generation, method, kernel
+// family and tiling, so repeated calls on one gather reuse the uploaded table.
+std::string work_key(const sb_gather* g, int method, int family, int ntiles, int npar, int pchunk) {
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "%p/%llu/%d/%d/%d/%d/%d", (const void*)g, (unsigned long long)g->serial, method,
+                family, ntiles, npar, pchunk);
+  return buf;
+}
+
+void base_args(const sb_gather* g, sb_result* r, SearchArgs& a) {
+  std::memset(&a, 0, sizeof a);
+  a.samples = g->d_samples;
+  a.scale = g->d_scale;
+  a.stride = g->stride;
+  a.cdp_off = g->d_cdp_off;
+  a.h_ref32 = g->d_h_ref32;
+  a.ncdp = g->ncdp;
+  a.ns = g->ns;
+  a.keys = r->keys;
+  a.out_index = r->index;
+  a.out_a = r->a;
+  a.out_b = r->b;
+  a.out_c = r->c;
+  a.out_coher = r->coher;
+  a.out_stack = r->stack;
+}
+
+int check_result(const sb_gather* g, const sb_result* r) {
+  if (r->ncdp != g->ncdp || r->ns != g->ns) return fail(SB_ERR_INVALID, "result size does not match gather");
+  if (r->ctx != g->ctx) return fail(SB_ERR_INVALID, "result and gather belong to different contexts");
+  return SB_OK;
+}
+
+int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int cc) {
+  // Enough blocks to fill every CU several times; chunks a multiple of 4*CC.
+  const int64_t want = (int64_t)ctx->num_cus * 8;
+  const int64_t have = (int64_t)ncdp * ntiles;
+  int nchunks = (int)std::min<int64_t>((want + have - 1) / have, (npar + 4 * cc - 1) / (4 * cc));
+  nchunks = std::max(1, nchunks);
+  int pchunk = (npar + nchunks - 1) / nchunks;
+  pchunk = (pchunk + 4 * cc - 1) / (4 * cc) * (4 * cc);
+  return pchunk;
+}
+
+int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int mode, double params_per_tuple,
+               const std::vector<int64_t>& S) {
+  HIPR(hipSetDevice(ctx->device));
+  std::lock_guard<std::mutex> lk(ctx->run_mu);
+  int rc;
+  int nblocks = 0;
+  const int store = g->store;
+  std::vector<int64_t> cost(g->ncdp);
+  double tuples = 0;
+  for (int i = 0; i < g->ncdp; i++) {
+    cost[i] = S[i];
+    tuples += (double)S[i] * g->ns * params_per_tuple;
+  }
+  int variant = -1;
+  if (mode == SB_MODE_FAST) {
+    if (!fast_supported(a.w)) return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
+    variant = ctx->fast_variant;
+    const FastVariant v = kFastVariants[variant];
+    const int tile = 64 * v.R;
+    const int ntiles = (g->ns + tile - 1) / tile;
+    const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override : choose_pchunk(ctx, g->ncdp, ntiles, a.npar, v.CC);
+    const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk);
+    if ((rc = build_work(ctx, cost, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+  } else {
+    const int ntiles = (g->ns + 63) / 64;
+    const std::string key = work_key(g, a.method, 1, ntiles, 1, 1);
+    if ((rc = build_work(ctx, cost, ntiles, 1, 1, &nblocks, key))) return rc;
+  }
+  a.work = (const WorkItem*)ctx->work_buf;
+  a.nwork = nblocks;
+  const int64_t nout = (int64_t)g->ncdp * g->ns;
+  HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
+  hipError_t e = hipSuccess;
+  if (mode == SB_MODE_FAST) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "fill");
+  HIPR(hipEventRecord(ctx->ev[1], ctx->stream));
+  if (mode == SB_MODE_FAST)
+    e = launch_search_fast(a, store, variant, nblocks, ctx->stream);
+  else
+    e = launch_search_exact(a, store, nblocks, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "search launch");
+  HIPR(hipEventRecord(ctx->ev[2], ctx->stream));
+  if (mode == SB_MODE_FAST) {
+    e = launch_finalize(a, store, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+  }
+  HIPR(hipEventRecord(ctx->ev[3], ctx->stream));
+  ctx->last = sb_timing{};
+  ctx->last.tuples = tuples;
+  ctx->last.grid_blocks = nblocks;
+  ctx->last.block_threads = mode == SB_MODE_FAST ? 256 : 64;
+  ctx->timing_pending = true;
+  return SB_OK;
+}
+
+}  // namespace
+
+extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_result* r) {
+  if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_cmp_run: null argument");
+  int rc;
+  if ((rc = check_result(g, r))) return rc;
+  if (p->nc <= 0) return fail(SB_ERR_INVALID, "nc must be positive");
+  if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
+  SearchArgs a;
+  base_args(g, r, a);
+  a.method = SB_METHOD_CMP;
+  a.slot_off = g->d_cdp_off;
+  a.slots = g->d_cmp_slots;
+  // CMP/OpenMP/src/main.cpp:68-110, all in float: c0 = stof(c0) * 1e6, ...
+  const float c0 = (float)((double)(float)p->c0 * 1e6);
+  const float c1 = (float)((double)(float)p->c1 * 1e6);
+  const float itau = (float)p->tau;
+  const float inc = (c1 - c0) * (1.0f / (float)p->nc);
+  float dt = (float)g->dt_us;
+  dt = dt / 1000000.0f;
+  const float idt = 1.0f / dt;
+  const float tt = itau * idt;
+  const int tau = (int)tt > 0 ? (int)tt : 0;
+  a.tau = tau;
+  a.w = 2 * tau + 1;
+  if (a.w > SB_MAX_W) return fail(SB_ERR_UNSUPPORTED, "window 2*tau+1 exceeds SB_MAX_W");
+  a.cf0 = c0;
+  a.cfinc = inc;
+  a.fc0 = c0;
+  a.finc_c = inc;
+  a.nc = p->nc;
+  a.na = a.nb = 1;
+  a.npar = p->nc;
+  a.dt_f = dt;
+  a.idt_f = idt;
+  std::vector<int64_t> S(g->ncdp);
+  for (int i = 0; i < g->ncdp; i++) S[i] = g->cdp_off[i + 1] - g->cdp_off[i];
+  return run_search(ctx, g, r, a, p->mode, (double)p->nc, S);
+}
+
+extern "C" int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_result* r) {
+  if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_crs_run: null argument");
+  int rc;
+  if ((rc = check_result(g, r))) return rc;
+  if (!g->has_crs) return fail(SB_ERR_STATE, "gather was created without neighbour lists");
+  if (p->na <= 0 || p->nb <= 0 || p->nc <= 0) return fail(SB_ERR_INVALID, "na, nb, nc must be positive");
+  if ((int64_t)p->na * p->nb * p->nc > INT32_MAX / 2) return fail(SB_ERR_INVALID, "parameter grid too large");
+  if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
+  SearchArgs a;
+  base_args(g, r, a);
+  a.method = SB_METHOD_CRS;
+  a.slot_off = g->d_crs_slot_off;
+  a.slots = g->d_crs_slots;
+  a.slots_d = g->d_crs_slots_d;
+  // CRS/OpenMP/src/main.cpp:85-138, all in double.
+  const double c0 = p->c0 * 1e6, c1 = p->c1 * 1e6;
+  const double inc_a = (p->a1 - p->a0) * (1.0 / (double)p->na);
+  const double inc_b = (p->b1 - p->b0) * (1.0 / (double)p->nb);
+  const double inc_c = (c1 - c0) * (1.0 / (double)p->nc);
+  double dt = (double)g->dt_us;
+  dt = dt / 1000000.0;
+  const double idt = 1.0 / dt;
+  const double tt = p->tau * idt;
+  const int tau = (int)tt > 0 ? (int)tt : 0;
+  a.tau = tau;
+  a.w = 2 * tau + 1;
+  if (a.w > SB_MAX_W) return fail(SB_ERR_UNSUPPORTED, "window 2*tau+1 exceeds SB_MAX_W");
+  a.na = p->na;
+  a.nb = p->nb;
+  a.nc = p->nc;
+  a.npar = p->na * p->nb * p->nc;
+  a.da0 = p->a0;
+  a.dinc_a = inc_a;
+  a.db0 = p->b0;
+  a.dinc_b = inc_b;
+  a.dc0 = c0;
+  a.dinc_c = inc_c;
+  a.fa0 = (float)p->a0;
+  a.finc_a = (float)inc_a;
+  a.fb0 = (float)p->b0;
+  a.finc_b = (float)inc_b;
+  a.fc0 = (float)c0;
+  a.finc_c = (float)inc_c;
+  a.dt_d = dt;
+  a.idt_d = idt;
+  std::vector<int64_t> S(g->ncdp);
+  for (int i = 0; i < g->ncdp; i++) S[i] = g->crs_slot_off[i + 1] - g->crs_slot_off[i];
+  return run_search(ctx, g, r, a, p->mode, (double)a.npar, S);
+}
+
+extern "C" int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_result* r) {
+  if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_crs_de_run: null argument");
+  int rc;
+  if ((rc = check_result(g, r))) return rc;
+  if (!g->has_crs) return fail(SB_ERR_STATE, "gather was created without neighbour lists");
+  if (p->pop < 4 || p->pop > 64) return fail(SB_ERR_INVALID, "population must be in [4, 64]");
+  if (p->ngen < 0) return fail(SB_ERR_INVALID, "ngen must be >= 0");
+  if (!(p->CR >= 0 && p->CR <= 1) || !(p->F >= 0 && p->F <= 2)) return fail(SB_ERR_INVALID, "bad F or CR");
+  if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
+  SearchArgs a;
+  base_args(g, r, a);
+  a.method = SB_METHOD_CRS;
+  a.slot_off = g->d_crs_slot_off;
+  a.slots = g->d_crs_slots;
+  a.slots_d = g->d_crs_slots_d;
+  a.slots_p = g->d_crs_slots_p;
+  // Window as the CRS reference computes it (double); fitness time base in
+  // float (the precision of the reference's FP32 CRS variants).
+  double dtd = (double)g->dt_us;
+  dtd = dtd / 1000000.0;
+  const double idtd = 1.0 / dtd;
+  const double tt = p->tau * idtd;
+  a.tau = (int)tt > 0 ? (int)tt : 0;
+  a.w = 2 * a.tau + 1;
+  if (a.w > SB_MAX_W) return fail(SB_ERR_UNSUPPORTED, "window 2*tau+1 exceeds SB_MAX_W");
+  if (p->mode == SB_MODE_FAST && !fast_supported(a.w))
+    return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
+  float dt = (float)g->dt_us;
+  dt = dt / 1000000.0f;
+  a.dt_f = dt;
+  a.idt_f = 1.0f / dt;
+  DeArgs d;
+  std::memset(&d, 0, sizeof d);
+  d.pop = p->pop;
+  d.ngen = p->ngen;
+  d.F = (float)p->F;
+  d.cr_thr = (uint32_t)(p->CR * 16777216.0);
+  d.seed = p->seed;
+  d.lo[0] = (float)p->a0;
+  d.hi[0] = (float)p->a1;
+  d.lo[1] = (float)p->b0;
+  d.hi[1] = (float)p->b1;
+  d.lo[2] = (float)(p->c0 * 1e6);
+  d.hi[2] = (float)(p->c1 * 1e6);
+  const int P2 = d.pop <= 4 ? 4 : d.pop <= 8 ? 8 : d.pop <= 16 ? 16 : d.pop <= 32 ? 32 : 64;
+  const int per_block = 256 / P2;
+  const int ntiles = (g->ns + per_block - 1) / per_block;
+
+  HIPR(hipSetDevice(ctx->device));
+  std::lock_guard<std::mutex> lk(ctx->run_mu);
+  std::vector<int64_t> cost(g->ncdp);
+  double tuples = 0;
+  for (int i = 0; i < g->ncdp; i++) {
+    cost[i] = g->crs_slot_off[i + 1] - g->crs_slot_off[i];
+    tuples += (double)cost[i] * g->ns * (double)p->pop * (p->ngen + 1);
+  }
+  int nblocks = 0;
+  if ((rc = build_work(ctx, cost, ntiles, 1, 1, &nblocks, work_key(g, SB_METHOD_CRS, 2, ntiles, 1, 1)))) return rc;
+  a.work = (const WorkItem*)ctx->work_buf;
+  a.nwork = nblocks;
+  HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPR(hipEventRecord(ctx->ev[1], ctx->stream));
+  hipError_t e = launch_de(a, d, g->store, p->mode == SB_MODE_EXACT, nblocks, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "CRS-DE launch");
+  HIPR(hipEventRecord(ctx->ev[2], ctx->stream));
+  HIPR(hipEventRecord(ctx->ev[3], ctx->stream));
+  ctx->last = sb_timing{};
+  ctx->last.tuples = tuples;
+  ctx->last.grid_blocks = nblocks;
+  ctx->last.block_threads = 256;
+  ctx->timing_pending = true;
+  return SB_OK;
+}
+
+extern "C" int sb_last_timing(const sb_ctx* cctx, sb_timing* t) {
+  sb_ctx* ctx = const_cast<sb_ctx*>(cctx);
+  if (!ctx || !t) return fail(SB_ERR_INVALID, "sb_last_timing: bad arguments");
+  if (ctx->timing_pending) {
+    HIPR(hipSetDevice(ctx->device));
+    HIPR(hipEventSynchronize(ctx->ev[3]));
+    float a = 0, b = 0, c = 0;
+    HIPR(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]));
+    HIPR(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
+    HIPR(hipEventElapsedTime(&c, ctx->ev[0], ctx->ev[3]));
+    ctx->last.search_ms = a;
+    ctx->last.finalize_ms = b;
+    ctx->last.total_ms = c;
+    ctx->timing_pending = false;
+  }
+  *t = ctx->last;
+  return SB_OK;
+}
+
+namespace {
+template <class Run, class P>
+int run_and_download(sb_ctx* ctx, const sb_gather* g, const P* p, sb_host_result* out, Run run) {
+  if (!ctx || !g || !p || !out) return fail(SB_ERR_INVALID, "null argument");
+  sb_result* r = nullptr;
+  int rc = sb_result_create(ctx, g->ncdp, g->ns, &r);
+  if (rc) return rc;
+  rc = run(ctx, g, p, r);
+  if (!rc) rc = sb_result_download(ctx, r, out);
+  sb_result_destroy(r);
+  return rc;
+}
+}  // namespace
+
+extern "C" int sb_cmp(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_host_result* out) {
+  return run_and_download(ctx, g, p, out, sb_cmp_run);
+}
+extern "C" int sb_crs(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_host_result* out) {
+  return run_and_download(ctx, g, p, out, sb_crs_run);
+}
+extern "C" int sb_crs_de(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_host_result* out) {
+  return run_and_download(ctx, g, p, out, sb_crs_de_run);
+}
+

@@ -1,0 +1,73 @@
+// Internal object layouts of the device runtime.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "semblance.h"
+
+struct sb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0, used = 0;
+  std::mutex mem_mu, run_mu;
+  std::map<void*, size_t> sizes;
+  hipEvent_t ev[4] = {};
+  sb_timing last{};
+  bool timing_pending = false;
+  int num_cus = 256;
+  // launch shape of the fast search (autotuned or set by sb_tune_set)
+  int fast_variant = 0;
+  int pchunk_override = 0;
+  std::map<std::string, std::pair<int, int>> tune_cache;
+  // reusable work table / queue counters
+  void* work_buf = nullptr;
+  size_t work_cap = 0;
+  std::string work_key;  // identifies the table in work_buf
+  int work_n = 0;
+  unsigned long long gather_serial = 0;
+  void* queue_buf = nullptr;
+};
+
+struct sb_gather {
+  sb_ctx* ctx = nullptr;
+  unsigned long long serial = 0;  // unique per ctx, keys cached work tables
+  int ncdp = 0, ntr = 0, ns = 0, dt_us = 0, store = 0;
+  int64_t stride = 0;
+  bool has_crs = false;
+  void* d_samples = nullptr;
+  float* d_scale = nullptr;
+  int32_t* d_cdp_off = nullptr;
+  float* d_h_ref32 = nullptr;
+  double *d_h0 = nullptr, *d_m0x = nullptr, *d_m0y = nullptr;
+  sb::SlotF* d_cmp_slots = nullptr;
+  int32_t* d_crs_slot_off = nullptr;
+  sb::SlotF* d_crs_slots = nullptr;
+  sb::SlotD* d_crs_slots_d = nullptr;
+  sb::SlotF* d_crs_slots_p = nullptr;  // unscaled FP32 geometry (CRS-DE exact mode)
+  int64_t ncrs_slots = 0;
+  std::vector<int32_t> cdp_off, crs_slot_off;  // host copies
+  size_t bytes = 0;
+};
+
+struct sb_result {
+  sb_ctx* ctx = nullptr;
+  int ncdp = 0, ns = 0;
+  void* base = nullptr;
+  int32_t* index = nullptr;
+  float *a = nullptr, *b = nullptr, *c = nullptr, *coher = nullptr, *stack = nullptr;
+  unsigned long long* keys = nullptr;
+};
+
+namespace sb {
+int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p);
+void ctx_free(sb_ctx* ctx, void* p);
+int hip_fail(hipError_t e, const char* what);
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int ntiles, int npar, int pchunk, int* nblocks,
+               const std::string& key);
+}  // namespace sb
