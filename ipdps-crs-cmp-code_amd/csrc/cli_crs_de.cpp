@@ -1,0 +1,80 @@
+// crs-de: the CRS search by differential evolution, with the flag surface the
+// reference documents for crs-ocl-de (README.md:88-106).  Outputs as crs.
+//
+//   -ngen    generations (population size from SB_DE_POP, default 32;
+//            F = SB_DE_F (0.5), CR = SB_DE_CR (0.9), seed = SB_DE_SEED (1))
+//   -d       device: 1-based index as in the reference; 0 = the first device
+//   -azimuth accepted for compatibility; the traveltime uses the unsigned
+//            midpoint distance of the CRS reference and does not depend on it
+//   -na/-nb/-nc accepted for compatibility; the search is continuous
+#include "cli_common.hpp"
+
+static double env_d(const char* k, double dflt) {
+  const char* v = std::getenv(k);
+  return v ? std::atof(v) : dflt;
+}
+
+int main(int argc, const char** argv) {
+  using namespace cli;
+  touch_outputs({"crs.a.su", "crs.b.su", "crs.c.su", "crs.coher.su", "crs.stack.su"});
+  Parser p;
+  p.add("-a0", "A0 constant");
+  p.add("-a1", "A1 constant");
+  p.add("-aph", "APH constant");
+  p.add("-apm", "APM constant");
+  p.add("-azimuth", "Azimuth angle");
+  p.add("-b0", "B0 constant");
+  p.add("-b1", "B1 constant");
+  p.add("-c0", "C0 constant");
+  p.add("-c1", "C1 constant");
+  p.add("-d", "OpenCL device");
+  p.add("-i", "Data path");
+  p.add("-na", "NA constant");
+  p.add("-nb", "NB constant");
+  p.add("-nc", "NC constant");
+  p.add("-ngen", "Tau constant");
+  p.add("-tau", "Tau constant");
+  p.add("-v", "Verbosity Level 0-3");
+  p.parse(argc, argv);
+
+  sb_de_params prm;
+  prm.a0 = to_d(p.get("-a0", true));
+  prm.a1 = to_d(p.get("-a1", true));
+  prm.b0 = to_d(p.get("-b0", true));
+  prm.b1 = to_d(p.get("-b1", true));
+  prm.c0 = to_d(p.get("-c0", true));
+  prm.c1 = to_d(p.get("-c1", true));
+  prm.tau = to_d(p.get("-tau", true));
+  (void)to_i(p.get("-na", true));
+  (void)to_i(p.get("-nb", true));
+  (void)to_i(p.get("-nc", true));
+  (void)to_d(p.get("-azimuth", true));
+  prm.ngen = to_i(p.get("-ngen", true));
+  const int aph = to_i(p.get("-aph", true));
+  const int apm = to_i(p.get("-apm", true));
+  const int d = to_i(p.get("-d", false));
+  const std::string path = p.get("-i", true);
+  set_verbosity(to_i(p.get("-v", false)));
+  prm.pop = (int)env_d("SB_DE_POP", 32);
+  prm.F = env_d("SB_DE_F", 0.5);
+  prm.CR = env_d("SB_DE_CR", 0.9);
+  prm.seed = (uint64_t)env_d("SB_DE_SEED", 1);
+  prm.mode = env_mode();
+
+  Session s;
+  s.load(path, SB_METHOD_CRS, aph, apm);
+  const double t0 = now_s();
+  s.upload(env_device(d > 0 ? d - 1 : 0), true);
+  HostOut out((size_t)s.ncdp * s.ns);
+  check(sb_crs_de(s.ctx, s.g, &prm, &out.view), "CRS-DE search");
+  const double t = now_s() - t0;
+  const double evals = (double)prm.pop * (prm.ngen + 1);
+  const double stps = (s.traces_per_eval(true) / 1e9) * (s.ns * evals / t);
+  log(INFO, "Execution Time: " + std::to_string(t) + ", Mega-Semblances-Trace/s: " + std::to_string(stps));
+  write(s, "crs.a.su", out.a, 1);
+  write(s, "crs.b.su", out.b, 1);
+  write(s, "crs.c.su", out.c, 1);
+  write(s, "crs.coher.su", out.coher, 0);
+  write(s, "crs.stack.su", out.stack, 0);
+  return EXIT_SUCCESS;
+}
