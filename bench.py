@@ -114,6 +114,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--stores", default="f32,f16", help="trace storage types to time")
+    ap.add_argument("--variant", type=int, default=-1, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--pchunk", type=int, default=0, help="parameters per workgroup with --variant")
+    ap.add_argument("--no-latency", action="store_true", help="skip the 1-CDP panel latency probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -128,10 +132,12 @@ def main():
     out = {}
     with sb.Context(0) as ctx:
         results = {}
-        for store in ("f32", "f16"):
+        for store in args.stores.split(","):
             print("bench: uploading gather (%s)" % store, flush=True)
             g = sb.Gather(ctx, cdp_off, geom, samples, CFG["dt_us"], store=sb.STORES[store])
-            if not args.no_tune:
+            if args.variant >= 0:
+                ctx.tune_set(args.variant, args.pchunk)
+            elif not args.no_tune:
                 v, pc, _ = sb.tune(ctx, g, prm)
                 print("bench: tuned variant %d pchunk %d (%s)" % (v, pc, store), flush=True)
             wall, tk = timed_steps(sb, ctx, g, prm, args.steps, args.warmup)
@@ -144,7 +150,10 @@ def main():
             print("bench: %s %.4g evals/s, search %.3f ms, finalize %.3f ms" %
                   (store, results[store]["value"], search_ms, fin_ms), flush=True)
             g.close()
-        p50, p99 = panel_latency(sb, ctx, cdp_off, geom, samples)
+        p50, p99 = (None, None) if args.no_latency else panel_latency(sb, ctx, cdp_off, geom, samples)
+    if "f32" not in results or "f16" not in results:
+        print(json.dumps({k: v for k, v in results.items()}), flush=True)
+        return
     f32 = results["f32"]
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
     traffic = None

@@ -129,7 +129,7 @@ template <int W, int STORE>
 __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, float pa, float pb, float pc, float& sem,
                              float& stk) {
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
-  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
   const int nrec = (int)(a.stride * ESZ);
   const char* __restrict__ base = (const char*)a.samples;
   float num[W], den = 0.f;
@@ -144,19 +144,15 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
         __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
     const float y = fmaf(pa, s.m, T0);
     const float tt = fmaf(y, y, fmaf(pb, s.m2, pc * s.h));
-    const float ts = __builtin_amdgcn_sqrtf(tt);
-    const bool ok = (ts >= lo) && (ts < hi);
+    // t^2 < 0 is invalid (the reference's t = -1): clamp to keep the
+    // fraction finite; the window test rejects it unless tau = 0.
+    const float ts = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.f));
     const int e = (int)ts - a.tau;
-    const float x = ok ? __builtin_amdgcn_fractf(ts) : 0.f;
-    unsigned off, sh;
-    if constexpr (STORE == SB_STORE_F32) {
-      off = ok ? (unsigned)e * 4u : kOob;
-      sh = 0;
-    } else {
-      off = ok ? ((unsigned)e & ~1u) * 2u : kOob;
-      sh = ((unsigned)e & 1u) << 4;
-    }
-    const Win<W + 1> sv = load_win<W + 1, STORE>(rs, off, sh);
+    bool ok = (unsigned)e < nwin;
+    if constexpr (W == 1) ok = ok && tt >= 0.f;
+    const float x = __builtin_amdgcn_fractf(ts);
+    const unsigned off = ok ? (STORE == SB_STORE_F32 ? (unsigned)e * 4u : ((unsigned)e & ~1u) * 2u) : kOob;
+    const Win<W + 1> sv = load_win<W + 1, STORE>(rs, off, ((unsigned)e & 1u) << 4);
     cnt += ok ? 1 : 0;
     if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll

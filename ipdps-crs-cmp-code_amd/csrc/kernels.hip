@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
 #pragma unroll
   for (int r = 0; r < R; r++) best[r] = 0ull;
 
-  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const unsigned nwin = (unsigned)max(a.ns - a.w, 0);  // number of valid window starts
   const int nbnc = a.nb * a.nc;
   const int nrec = (int)(a.stride * ESZ);
   const char* __restrict__ base = (const char*)a.samples;
@@ -508,25 +508,43 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
           } else {
             tt = T0sq[r] + U[ci];
           }
-          const float ts = __builtin_amdgcn_sqrtf(tt);
-          const bool ok = (ts >= lo) && (ts < hi);  // false for NaN (tt < 0)
+          float ts;
+          bool ok;
+          if constexpr (CRS) {
+            // t^2 < 0 is invalid (the reference sets t = -1): clamp so the
+            // fraction stays finite, then the window test rejects it unless
+            // tau = 0, where the sign test does.
+            ts = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.f));
+          } else {
+            ts = __builtin_amdgcn_sqrtf(tt);  // tt >= 0: the host checks c0, c1 >= 0
+          }
+          // Window start e = it - tau must lie in [0, ns - w): the
+          // reference's it - tau >= 0 && it + tau + 1 < ns.
           const int e = (int)ts - a.tau;
+          ok = (unsigned)e < nwin;
+          if constexpr (CRS && W == 1) ok = ok && tt >= 0.f;
           okk[r][ci] = ok;
-          xx[r][ci] = ok ? __builtin_amdgcn_fractf(ts) : 0.f;
+          xx[r][ci] = __builtin_amdgcn_fractf(ts);
           if constexpr (STORE == SB_STORE_F32) {
             off[r][ci] = ok ? (unsigned)e * 4u : kOob;
-            shf[r][ci] = 0;
           } else {
             off[r][ci] = ok ? ((unsigned)e & ~1u) * 2u : kOob;
             shf[r][ci] = ((unsigned)e & 1u) << 4;
           }
         }
       }
+      constexpr bool HALF = STORE == SB_STORE_F16 && W <= 7;
       Win<W + 1> sv[R][CC];
+      WinH<W + 1> sh[R][CC];
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int ci = 0; ci < CC; ci++) sv[r][ci] = load_win<W + 1, STORE>(rs, off[r][ci], shf[r][ci]);
+        for (int ci = 0; ci < CC; ci++) {
+          if constexpr (HALF)
+            sh[r][ci] = load_win_h<W + 1>(rs, off[r][ci], shf[r][ci]);
+          else
+            sv[r][ci] = load_win<W + 1, STORE>(rs, off[r][ci], shf[r][ci]);
+        }
 #pragma unroll
       for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -540,6 +558,18 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
               num[r][ci][j] += v;
               den[r][ci] = fmaf(v, v, den[r][ci]);
             }
+          } else if constexpr (HALF) {
+            // v = s_j + x s_{j+1} - x s_j: two mixed-precision FMAs
+            // (v_fma_mix_f32) per sample, halves read in place.
+            float q = 0.f;
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+              const float hj = (float)sh[r][ci].h[j], hj1 = (float)sh[r][ci].h[j + 1];
+              const float v = fmaf(-x, hj, fmaf(x, hj1, hj));
+              num[r][ci][j] = fmaf(s.scale, v, num[r][ci][j]);
+              q = fmaf(v, v, q);
+            }
+            den[r][ci] = fmaf(s.scale2, q, den[r][ci]);
           } else {
             float q = 0.f;
 #pragma unroll

@@ -548,7 +548,10 @@ extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* 
   a.idt_f = idt;
   std::vector<int64_t> S(g->ncdp);
   for (int i = 0; i < g->ncdp; i++) S[i] = g->cdp_off[i + 1] - g->cdp_off[i];
-  return run_search(ctx, g, r, a, p->mode, (double)p->nc, S);
+  // The fast kernel assumes t0^2 + C h^2 >= 0; a negative C range (NaN
+  // traveltimes, all invalid in the reference) runs in reference arithmetic.
+  const int mode = (c0 < 0.f || c1 < 0.f) ? SB_MODE_EXACT : p->mode;
+  return run_search(ctx, g, r, a, mode, (double)p->nc, S);
 }
 
 extern "C" int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_result* r) {

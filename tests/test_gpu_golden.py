@@ -33,10 +33,18 @@ def _crs_gather(sb, case, p):
     return sb.SuGather(os.path.join(GOLD, case, "input.su"), sb.METHOD_CRS, p["aph"], p["apm"])
 
 
-def near_tie(panel_row, pick, rel=1e-5):
-    """True when the oracle's best and the picked semblance are within rel."""
-    best = float(np.max(panel_row))
-    return abs(best - float(panel_row[pick])) <= rel * max(abs(best), 1e-30)
+def near_tie(row, pick, tol=1e-5):
+    """Top-two semblances (values in [0, 1]) within tol of each other."""
+    return pick >= 0 and float(np.max(row)) - float(row[pick]) <= tol
+
+
+def explained(row32, row64, ref_pick, our_pick, tol=1e-5):
+    """A differing pick is a near-tie of the reference's own values, or the
+    FP64 oracle ranks it at least as high as the reference's pick (the
+    reference's FP32 rounding decided)."""
+    if our_pick < 0 or ref_pick < 0:
+        return False
+    return near_tie(row32, our_pick, tol) or row64[our_pick] >= row64[ref_pick] - 1e-9
 
 
 @pytest.mark.parametrize("case", CMP_CASES)
@@ -74,13 +82,14 @@ def test_cmp_fast_matches_reference(sb, orc, gpu_ctx, manifest, case):
     p = cmp_args(manifest[case])
     G = _cmp_gather(sb, case, p)
     ref = orc.cmp(G.cdp_off, G.geom, G.samples, G.dt_us, p["c0"], p["c1"], p["nc"], p["tau"], panel=True)
+    ref64 = orc.cmp_f64(G.cdp_off, G.geom, G.samples, G.dt_us, p["c0"], p["c1"], p["nc"], p["tau"], panel=True)
     dg = sb.Gather(gpu_ctx, G.cdp_off, G.geom, G.samples, G.dt_us)
     out = sb.search(gpu_ctx, dg, sb.cmp_params(p["c0"], p["c1"], p["nc"], p["tau"], sb.MODE_FAST))
     dg.close()
     diff = np.argwhere(out["index"] != ref["index"])
     for cdp, t0 in diff:
-        row = ref["panel_sem"][cdp, t0]
-        assert out["index"][cdp, t0] >= 0 and near_tie(row, out["index"][cdp, t0]), (case, cdp, t0)
+        assert explained(ref["panel_sem"][cdp, t0], ref64["panel_sem"][cdp, t0], ref["index"][cdp, t0],
+                         out["index"][cdp, t0]), (case, cdp, t0)
     same = out["index"] == ref["index"]
     for k in ("c", "coher", "stack"):
         assert np.array_equal(out[k][same], ref[k][same]), (case, k)
@@ -98,10 +107,10 @@ def test_crs_fast_matches_reference(sb, orc, gpu_ctx, manifest, case):
                         sb.MODE_FAST)
     out = sb.search(gpu_ctx, dg, prm)
     dg.close()
+    # CRS reference values are double: a differing pick must be a near-tie of them.
     diff = np.argwhere(out["index"] != ref["index"])
     for cdp, t0 in diff:
-        row = ref["panel_sem"][cdp, t0]
-        assert near_tie(row, out["index"][cdp, t0]), (case, cdp, t0)
+        assert near_tie(ref["panel_sem"][cdp, t0], out["index"][cdp, t0]), (case, cdp, t0)
     same = out["index"] == ref["index"]
     for k in ("a", "b", "c", "coher", "stack"):
         assert np.array_equal(out[k][same], ref[k][same]), (case, k)
