@@ -28,6 +28,14 @@ FLOP_PER_TUPLE_CMP = 5 + 7 * 3        # traveltime 5 + 7 per window sample, w = 
 PEAK_FP32_VECTOR_TFLOPS = 157.3       # MI355X published FP32 vector peak
 CFG = dict(ncdp=2000, fold=(30, 120), ns=1501, dt_us=2000, h_max=1500.0, h_jitter=5.0, nevents=8, snr=5.0,
            c0=1.98e-7, c1=1.77e-6, nc=256, tau=0.002, aph=2000)
+FLOP_PER_TUPLE_CRS = 10 + 7 * 3
+BOX = dict(a0=-0.7e-3, a1=0.7e-3, b0=-1e-7, b1=1e-7, c0=1.975e-7, c1=1.384e-6)
+# CRS grid: 500 CDPs, 1000 traces each (midpoints +-50 m, half-offsets 0-2600 m), aph 600, apm 50, 11^3 grid.
+CRS_CFG = dict(ncdp=500, fold=(1000, 1000), ns=1001, dt_us=4000, h_max=2600.0, mid=50.0, nevents=6, snr=3.0,
+               aph=600, apm=50, n=11, tau=0.004)
+# CRS-DE: 2000 CDPs of fold 200-1000, otherwise as the CRS grid; population 32, 30 generations.
+DE_CFG = dict(ncdp=2000, fold=(200, 1000), ns=1001, dt_us=4000, h_max=2600.0, mid=50.0, nevents=6, snr=3.0,
+              aph=600, apm=50, tau=0.004, pop=32, ngen=30, F=0.5, CR=0.9)
 
 
 def _load(name, path):
@@ -107,6 +115,119 @@ def cpu_baseline(sb, cdp_off, geom, samples, every=4):
         return dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port", sample=sample)
 
 
+def crs_workload(sb, cfg, seed):
+    cdp_off, geom, samples = sb.synth(1, cfg["ncdp"], cfg["fold"], cfg["ns"], cfg["dt_us"], seed, h_max=cfg["h_max"],
+                                      mid_half_width=cfg["mid"], nevents=cfg["nevents"], snr=cfg["snr"],
+                                      aph=cfg["aph"])
+    nbr_off, nbr = sb.neighbors(geom[cdp_off[:-1]], cfg["apm"])
+    return cdp_off, geom, samples, nbr_off, nbr
+
+
+def crs_cpu_baseline(sb, cdp_off, geom, samples, nbr_off, nbr, ncdp_sample):
+    """The reference crs-omp2 on the first ncdp_sample CDPs of the line (their
+    supergathers restricted to that range), all host cores, its INFO line."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "crs-omp2")
+    cores = len(os.sched_getaffinity(0))
+    n = ncdp_sample
+    rows = np.arange(cdp_off[0], cdp_off[n])
+    sample = "first %d CDPs (%d traces), full ns and grid" % (n, len(rows))
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "sample.su")
+        sb.write_traces(path, geom[rows], samples[rows], CRS_CFG["dt_us"])
+        b = BOX
+        cmd = [ref, "-a0", repr(b["a0"]), "-a1", repr(b["a1"]), "-na", "11", "-b0", repr(b["b0"]), "-b1", repr(b["b1"]),
+               "-nb", "11", "-c0", repr(b["c0"]), "-c1", repr(b["c1"]), "-nc", "11", "-aph", str(CRS_CFG["aph"]),
+               "-apm", str(CRS_CFG["apm"]), "-tau", repr(CRS_CFG["tau"]), "-i", path, "-v", "2"]
+        p = subprocess.run(cmd, cwd=td, capture_output=True, text=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close"))
+        m = re.search(r"Semblances-Trace/s: ([0-9.eE+-]+)", p.stdout)
+        if p.returncode != 0 or not m:
+            return None
+        return dict(value=float(m.group(1)) * 1e9, unit="semblance evals/s", cores=cores, kind="reference",
+                    sample=sample + "; reference INFO line")
+
+
+def de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, cdps):
+    """The oracle's CPU twin of CRS-DE (OpenMP, FP32) on the given CDPs."""
+    cores = len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    # restrict to the CDPs' neighbourhoods: keep every CDP, search all, time per tuple
+    c = DE_CFG
+    lo = min(cdps)
+    hi = max(cdps) + 1
+    sub_off = (cdp_off[lo:hi + 1] - cdp_off[lo]).astype(np.int32)
+    rows = np.arange(cdp_off[lo], cdp_off[hi])
+    # neighbour lists of the subrange, clipped to it
+    no, nb = [0], []
+    for i in range(lo, hi):
+        lst = [j - lo for j in nbr[nbr_off[i]:nbr_off[i + 1]] if lo <= j < hi]
+        nb += lst
+        no.append(len(nb))
+    no, nb = np.array(no, np.int32), np.array(nb, np.int32)
+    folds = np.diff(sub_off)
+    tuples = sum(float(sum(folds[j] for j in nb[no[i]:no[i + 1]])) for i in range(hi - lo)) * c["ns"] * c["pop"] * (
+        c["ngen"] + 1)
+    t = time.perf_counter()
+    orc.crs_de(sub_off, geom[rows], samples[rows], c["dt_us"], no, nb, BOX["a0"], BOX["a1"], BOX["b0"], BOX["b1"],
+               BOX["c0"], BOX["c1"], c["tau"], c["pop"], c["ngen"], c["F"], c["CR"], 1)
+    wall = time.perf_counter() - t
+    return dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port",
+                sample="CDPs %d-%d (%.3g evals), oracle CPU twin of CRS-DE, FP32" % (lo, hi - 1, tuples))
+
+
+def side_configs(sb, args):
+    """CRS grid (500 CDPs) and CRS-DE (2000 CDPs) at their full sizes."""
+    out = {}
+    orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+    print("bench: generating CRS grid workload", flush=True)
+    w = crs_workload(sb, CRS_CFG, SEED + 2)
+    cdp_off, geom, samples, nbr_off, nbr = w
+    b = BOX
+    prm = sb.crs_params(b["a0"], b["a1"], 11, b["b0"], b["b1"], 11, b["c0"], b["c1"], 11, CRS_CFG["tau"],
+                        sb.MODE_FAST)
+    with sb.Context(0) as ctx:
+        g = sb.Gather(ctx, cdp_off, geom, samples, CRS_CFG["dt_us"], nbr_off=nbr_off, nbr=nbr)
+        if not args.no_tune:
+            sb.tune(ctx, g, prm, reps=1)
+        wall, tk = timed_steps(sb, ctx, g, prm, 3, 1)
+        ms = float(np.mean([t["search_ms"] for t in tk]))
+        tuples = tk[0]["tuples"]
+        out["crs_grid"] = dict(workload="CRS 500 CDPs fold 1000 (aph 600 -> %d traces) ns 1001 11x11x11 apm 50 FP32" %
+                               len(geom), value=tuples * 3 / wall, ms_per_step=wall / 3 * 1e3, tuples=tuples,
+                               search_ms=ms, roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms * 1e-3) / 1e12 /
+                               PEAK_FP32_VECTOR_TFLOPS)
+        g.close()
+    if not args.no_cpu:
+        out["crs_grid"]["cpu_baseline"] = crs_cpu_baseline(sb, cdp_off, geom, samples, nbr_off, nbr, 6)
+    del w, cdp_off, geom, samples
+    print("bench: generating CRS-DE workload", flush=True)
+    cdp_off, geom, samples, nbr_off, nbr = crs_workload(sb, DE_CFG, SEED + 3)
+    c = DE_CFG
+    out["crs_de"] = dict(workload="CRS-DE 2000 CDPs fold 200-1000 (aph 600 -> %d traces) ns 1001 pop 32 ngen 30" %
+                         len(geom))
+    with sb.Context(0) as ctx:
+        for store in ("f16", "f32"):
+            g = sb.Gather(ctx, cdp_off, geom, samples, c["dt_us"], store=sb.STORES[store], nbr_off=nbr_off, nbr=nbr)
+            prm = sb.de_params(b["a0"], b["a1"], b["b0"], b["b1"], b["c0"], b["c1"], c["tau"], c["pop"], c["ngen"],
+                               c["F"], c["CR"], 1, sb.MODE_FAST)
+            wall, tk = timed_steps(sb, ctx, g, prm, 2, 1)
+            ms = float(np.mean([t["search_ms"] for t in tk]))
+            tuples = tk[0]["tuples"]
+            out["crs_de"][store] = dict(value=tuples * 2 / wall, ms_per_step=wall / 2 * 1e3, tuples=tuples,
+                                        search_ms=ms, roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms * 1e-3) /
+                                        1e12 / PEAK_FP32_VECTOR_TFLOPS)
+            print("bench: crs-de %s %.4g evals/s (%.1f ms)" % (store, out["crs_de"][store]["value"], ms), flush=True)
+            g.close()
+    if not args.no_cpu:
+        cpu = de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, [100, 101, 102, 103])
+        out["crs_de"]["cpu_baseline"] = cpu
+        out["crs_de"]["f16_vs_cpu"] = out["crs_de"]["f16"]["value"] / cpu["value"]
+    out["crs_de"]["f16_vs_f32"] = out["crs_de"]["f16"]["value"] / out["crs_de"]["f32"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +239,7 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="fixed launch shape (skips the autotuner)")
     ap.add_argument("--pchunk", type=int, default=0, help="parameters per workgroup with --variant")
     ap.add_argument("--no-latency", action="store_true", help="skip the 1-CDP panel latency probe")
+    ap.add_argument("--no-side", action="store_true", help="skip the CRS grid and CRS-DE side measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -188,6 +310,9 @@ def main():
     if not args.no_cpu:
         print("bench: CPU baseline", flush=True)
         line["cpu_baseline"] = cpu_baseline(sb, cdp_off, geom, samples)
+    if not args.no_side:
+        del cdp_off, geom, samples
+        line.update(side_configs(sb, args))
     print(json.dumps(line), flush=True)
 
 
