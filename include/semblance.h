@@ -258,9 +258,15 @@ int sb_synth(const sb_synth_spec* spec, int* ncdp, int* ntr, int32_t* cdp_off,
              sb_trace_geom* geom, float* samples);
 
 /* ------------------------------------------------------------- serving */
-/* A resident request loop for interactive velocity analysis: requests of
- * uneven fold / length are admitted while the HBM budget allows, their
- * gathers stay on the device, and admitted requests are batched per launch. */
+/* A resident request loop for interactive velocity analysis (continuous
+ * batching on one GPU).  Worker slots, each with its own HIP stream, take
+ * requests in arrival order as soon as a slot is idle and the request's
+ * device footprint fits the free HBM budget, so requests that arrive while
+ * others run are admitted as memory frees up and their kernels overlap on
+ * the device; waiting requests of the same shape (method, ns, dt, parameters,
+ * store) are merged into one gather and one launch.  A request's results do
+ * not depend on what it was batched with.  No reference counterpart: the
+ * reference is a batch CLI. */
 typedef struct sb_server sb_server;
 
 typedef struct {
@@ -281,14 +287,19 @@ typedef struct {
   int status;
 } sb_request_stats;
 
+/* hbm_budget: bytes the server may hold (0 = 90% of free HBM); max_batch:
+ * most CDPs merged into one launch. */
 int sb_server_create(int device, size_t hbm_budget, int max_batch, sb_server** out);
-/* Copies the request; returns a ticket.  Never blocks on the GPU. */
+/* Queues a request and returns its ticket without blocking.  The request's
+ * arrays are read in place: keep them valid until sb_server_wait returns. */
 int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* ticket);
 /* Blocks until the ticket is done (timeout_ms < 0: forever); copies results
- * into out (arrays [ncdp][ns]) and releases the request. */
+ * into out (arrays [ncdp][ns]) and releases the request.  A request larger
+ * than the whole budget fails with SB_ERR_BUDGET. */
 int sb_server_wait(sb_server* s, uint64_t ticket, double timeout_ms,
                    sb_host_result* out, sb_request_stats* st);
 int sb_server_stats(const sb_server* s, uint64_t* done, uint64_t* launches, size_t* hbm_used);
+/* Stops the workers; queued requests that have not started fail. */
 void sb_server_destroy(sb_server* s);
 
 #ifdef __cplusplus

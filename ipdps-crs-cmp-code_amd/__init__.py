@@ -367,3 +367,65 @@ def tune(ctx, gather, params, reps=2):
     method = METHOD_CMP if isinstance(params, CmpParams) else METHOD_CRS
     check(lib().sb_tune(ctx.h, gather.h, method, C.cast(C.pointer(params), C.c_void_p), reps))
     return ctx.tune_get()
+
+
+class Server:
+    """sb_server: resident request loop.  Arrays passed to submit() are kept
+    referenced here until wait() returns (the C side reads them in place)."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch=64):
+        self.h = C.c_void_p()
+        check(lib().sb_server_create(device, hbm_budget, max_batch, C.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, cdp_off, geom, samples, dt_us, params, store=STORE_F32, nbr_off=None, nbr=None):
+        off = _i32(cdp_off)
+        g = np.ascontiguousarray(geom, GEOM_DTYPE)
+        s = np.ascontiguousarray(samples, np.float32)
+        req = Request()
+        req.ncdp, req.ns, req.dt_us, req.store = len(off) - 1, s.shape[1], dt_us, store
+        req.cdp_off = _ptr(off, C.c_int32)
+        req.geom = g.ctypes.data
+        req.samples = _ptr(s, C.c_float)
+        keep = [off, g, s]
+        if isinstance(params, CmpParams):
+            req.method, req.cmp = METHOD_CMP, params
+        else:
+            req.method, req.crs = METHOD_CRS, params
+            no, nb = _i32(nbr_off), _i32(nbr if len(nbr) else [0])
+            req.nbr_off, req.nbr = _ptr(no, C.c_int32), _ptr(nb, C.c_int32)
+            keep += [no, nb]
+        t = C.c_uint64()
+        check(lib().sb_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._keep[t.value] = (keep, req.ncdp, req.ns)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        keep, ncdp, ns = self._keep[ticket]
+        out = {k: np.zeros((ncdp, ns), np.int32 if k == "index" else np.float32)
+               for k in ("index", "a", "b", "c", "coher", "stack")}
+        hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float)
+                          for k in ("index", "a", "b", "c", "coher", "stack")])
+        st = RequestStats()
+        rc = lib().sb_server_wait(self.h, ticket, timeout_ms, C.byref(hr), C.byref(st))
+        self._keep.pop(ticket, None)
+        stats = {k: getattr(st, k) for k, _ in RequestStats._fields_}
+        if rc != 0:
+            return None, dict(stats, error=lib().sb_strerror(rc).decode())
+        return out, stats
+
+    def stats(self):
+        d, l, u = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        check(lib().sb_server_stats(self.h, C.byref(d), C.byref(l), C.byref(u)))
+        return dict(done=d.value, launches=l.value, hbm_reserved=u.value)
+
+    def close(self):
+        if self.h:
+            lib().sb_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     T0[r] = (float)(wi.tile * 64 * R + r * 64 + lane);
-    T0sq[r] = T0[r] * T0[r];
+    T0sq[r] = __fmul_rn(T0[r], T0[r]);
   }
   // Contiguous parameter sub-range of this wave, a multiple of CC long.
   const int np = wi.p1 - wi.p0;
@@ -483,16 +483,15 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
       nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
       const __amdgpu_buffer_rsrc_t rs =
           __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
-      float U[CC], V[CC];
+      // Every rounding step below is explicit (fmaf / __fmul_rn), so an
+      // evaluation's bits do not depend on the launch shape or batch.
+      float U[CC];
 #pragma unroll
       for (int ci = 0; ci < CC; ci++) {
-        if constexpr (CRS) {
-          V[ci] = pa[ci] * s.m;
-          U[ci] = fmaf(pb[ci], s.m2, pc[ci] * s.h);
-        } else {
-          V[ci] = 0.f;
-          U[ci] = pc[ci] * s.h;
-        }
+        if constexpr (CRS)
+          U[ci] = fmaf(pb[ci], s.m2, __fmul_rn(pc[ci], s.h));
+        else
+          U[ci] = 0.f;
       }
       unsigned off[R][CC], shf[R][CC];
       float xx[R][CC];
@@ -503,10 +502,10 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
         for (int ci = 0; ci < CC; ci++) {
           float tt;
           if constexpr (CRS) {
-            const float y = T0[r] + V[ci];
+            const float y = fmaf(pa[ci], s.m, T0[r]);
             tt = fmaf(y, y, U[ci]);
           } else {
-            tt = T0sq[r] + U[ci];
+            tt = fmaf(pc[ci], s.h, T0sq[r]);
           }
           float ts;
           bool ok;
@@ -554,8 +553,8 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
           if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll
             for (int j = 0; j < W; j++) {
-              const float v = fmaf(sv[r][ci].v[j + 1] - sv[r][ci].v[j], x, sv[r][ci].v[j]);
-              num[r][ci][j] += v;
+              const float v = fmaf(__fsub_rn(sv[r][ci].v[j + 1], sv[r][ci].v[j]), x, sv[r][ci].v[j]);
+              num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
               den[r][ci] = fmaf(v, v, den[r][ci]);
             }
           } else if constexpr (HALF) {
@@ -574,7 +573,7 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
             float q = 0.f;
 #pragma unroll
             for (int j = 0; j < W; j++) {
-              const float v = fmaf(sv[r][ci].v[j + 1] - sv[r][ci].v[j], x, sv[r][ci].v[j]);
+              const float v = fmaf(__fsub_rn(sv[r][ci].v[j + 1], sv[r][ci].v[j]), x, sv[r][ci].v[j]);
               num[r][ci][j] = fmaf(s.scale, v, num[r][ci][j]);
               q = fmaf(v, v, q);
             }
@@ -592,7 +591,7 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
         for (int j = 0; j < W; j++) acsq = fmaf(num[r][ci][j], num[r][ci][j], acsq);
         const float m = (float)cnt[r][ci];
         const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && (S - cnt[r][ci]) < 2;
-        const float sem = acsq * __builtin_amdgcn_rcpf(den[r][ci] * m);
+        const float sem = __fmul_rn(acsq, __builtin_amdgcn_rcpf(__fmul_rn(den[r][ci], m)));
         if (valid && sem > 0.f) {
           const unsigned long long key =
               ((unsigned long long)__float_as_uint(sem) << 32) | (unsigned long long)(~(uint32_t)(p + ci));
