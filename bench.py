@@ -228,6 +228,80 @@ def side_configs(sb, args):
     return out
 
 
+def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4):
+    """Serving loop under load.  A CMP line (96 CDPs of fold 24-1000, kept at
+    ns 501/1001/1501/2001/3001) and a CRS line (32 CDPs, ns 501/1001) are made
+    resident on the server once; requests name a line and a CDP range.
+    Poisson arrivals of CMP (70 %) and CRS (30 %) panel requests plus a probe
+    stream of 1-CDP CMP panels (ns 1501, nc 256); latency = server done -
+    submit time.  CMP requests: 1-64 CDPs, nc 21-128.  CRS requests: 1-8 CDPs,
+    5-11 values per parameter (the 21-128 per parameter of the full
+    specification offers ~10^3 times one GPU's capacity at 60 CRS
+    requests/s)."""
+    rng = np.random.default_rng(seed)
+    print("bench: generating serving lines", flush=True)
+    cmp_off, cmp_geom, cmp_s = sb.synth(0, 96, (24, 1000), 3001, 2000, seed, h_max=1500.0, h_jitter=5.0, nevents=8)
+    crs_off, crs_geom, crs_s = sb.synth(1, 32, (24, 1000), 1001, 4000, seed + 1, h_max=800.0, mid_half_width=50.0,
+                                        nevents=6)
+    crs_no, crs_nb = sb.neighbors(crs_geom[crs_off[:-1]], 50)
+    b = BOX
+    events = []
+    t = 0.0
+    while t < seconds:
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < 0.7:
+            n = int(rng.integers(1, 65))
+            i0 = int(rng.integers(0, 96 - n + 1))
+            prm = sb.cmp_params(CFG["c0"], CFG["c1"], int(rng.integers(21, 129)), 0.002)
+            events.append((t, "cmp", ("cmp", int(rng.choice([501, 1001, 2001, 3001]))), i0, i0 + n, prm))
+        else:
+            n = int(rng.integers(1, 9))
+            i0 = int(rng.integers(0, 32 - n + 1))
+            n3 = [int(x) for x in rng.integers(5, 12, 3)]
+            prm = sb.crs_params(b["a0"], b["a1"], n3[0], b["b0"], b["b1"], n3[1], b["c0"], b["c1"], n3[2], 0.004)
+            events.append((t, "crs", ("crs", int(rng.choice([501, 1001]))), i0, i0 + n, prm))
+    t = 0.0
+    while t < seconds:
+        t += rng.exponential(1.0 / probe_rate)
+        i0 = int(rng.integers(0, 96))
+        events.append((t, "probe", ("cmp", 1501), i0, i0 + 1, sb.cmp_params(CFG["c0"], CFG["c1"], 256, 0.002)))
+    events.sort(key=lambda e: e[0])
+    tickets = []
+    with sb.Server(0, max_batch=64) as srv:
+        gid = {}
+        for n in (501, 1001, 1501, 2001, 3001):
+            gid[("cmp", n)] = srv.add_gather(cmp_off, cmp_geom, np.ascontiguousarray(cmp_s[:, :n]), 2000)
+        for n in (501, 1001):
+            gid[("crs", n)] = srv.add_gather(crs_off, crs_geom, np.ascontiguousarray(crs_s[:, :n]), 4000,
+                                             nbr_off=crs_no, nbr=crs_nb)
+        resident = srv.stats()["hbm_reserved"]
+        # warm-up: one request of each kind
+        for ev in events[:4]:
+            srv.wait(srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5]))
+        print("bench: serving %d requests over %.1f s" % (len(events), seconds), flush=True)
+        start = time.perf_counter()
+        for ev in events:
+            d = ev[0] - (time.perf_counter() - start)
+            if d > 0:
+                time.sleep(d)
+            tickets.append((ev[1], srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5])))
+        lat = {"cmp": [], "crs": [], "probe": []}
+        for kind, tk in tickets:
+            out, st = srv.wait(tk)
+            if out is not None:
+                lat[kind].append(st["t_done_ms"] - st["t_submit_ms"])
+        wall = time.perf_counter() - start
+        stats = srv.stats()
+    res = dict(offered_rps=rate + probe_rate, duration_s=round(wall, 3), served=stats["done"] - 4,
+               launches=stats["launches"] - 4, resident_gb=round(resident / 1e9, 2))
+    for k, v in lat.items():
+        v = np.array(v)
+        if len(v):
+            res[k] = dict(n=len(v), p50_ms=float(np.percentile(v, 50)), p99_ms=float(np.percentile(v, 99)),
+                          max_ms=float(v.max()))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +314,7 @@ def main():
     ap.add_argument("--pchunk", type=int, default=0, help="parameters per workgroup with --variant")
     ap.add_argument("--no-latency", action="store_true", help="skip the 1-CDP panel latency probe")
     ap.add_argument("--no-side", action="store_true", help="skip the CRS grid and CRS-DE side measurements")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop load test")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -313,6 +388,10 @@ def main():
     if not args.no_side:
         del cdp_off, geom, samples
         line.update(side_configs(sb, args))
+    if not args.no_serving:
+        line["serving"] = serving_bench(sb)
+        if "probe" in line["serving"]:
+            line["panel_latency_ms"]["p99_under_load"] = line["serving"]["probe"]["p99_ms"]
     print(json.dumps(line), flush=True)
 
 

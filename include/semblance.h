@@ -203,6 +203,17 @@ int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_resul
 /* CRS-DE search (described only in README.md:82-131 of the reference). */
 int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_result* r);
 
+/* The same searches over CDPs [cdp_begin, cdp_end) of a resident gather
+ * (velocity panels); r holds (cdp_end - cdp_begin) x ns results.  The gather
+ * may belong to another ctx of the same device.  No reference counterpart
+ * (the reference always searches whole files). */
+int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cdp_begin, int cdp_end,
+                     sb_result* r);
+int sb_crs_run_range(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, int cdp_begin, int cdp_end,
+                     sb_result* r);
+int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, int cdp_begin, int cdp_end,
+                        sb_result* r);
+
 /* Convenience: search and download in one call. */
 int sb_cmp(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_host_result* out);
 int sb_crs(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_host_result* out);
@@ -269,6 +280,12 @@ int sb_synth(const sb_synth_spec* spec, int* ncdp, int* ntr, int32_t* cdp_off,
  * reference is a batch CLI. */
 typedef struct sb_server sb_server;
 
+/* A request either carries its traces (gather_id == 0: cdp_off, geom,
+ * samples, nbr_off/nbr and store describe a binned panel, uploaded for this
+ * request) or names a resident gather (gather_id from sb_server_add_gather)
+ * and the CDPs [cdp_begin, cdp_end) of it to search; then only method and
+ * the parameters are read besides those three fields, and the results have
+ * cdp_end - cdp_begin rows. */
 typedef struct {
   int method;              /* SB_METHOD_CMP or SB_METHOD_CRS               */
   int ncdp, ns, dt_us;
@@ -280,6 +297,8 @@ typedef struct {
   sb_cmp_params cmp;
   sb_crs_params crs;
   int store;
+  int gather_id;           /* 0: traces carried above; else resident gather */
+  int cdp_begin, cdp_end;  /* CDP range of the resident gather              */
 } sb_request;
 
 typedef struct {
@@ -298,6 +317,16 @@ int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* ticket);
  * than the whole budget fails with SB_ERR_BUDGET. */
 int sb_server_wait(sb_server* s, uint64_t ticket, double timeout_ms,
                    sb_host_result* out, sb_request_stats* st);
+/* Uploads a binned line once and keeps it on the device (arguments as for
+ * sb_gather_create); requests then name it by *gather_id (> 0) with a CDP
+ * range.  Its bytes count against the server budget until it is removed. */
+int sb_server_add_gather(sb_server* s, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
+                         const sb_trace_geom* geom, const float* samples, int store,
+                         const int32_t* nbr_off, const int32_t* nbr, int* gather_id);
+/* Rejects new requests naming the gather, waits for those queued or running,
+ * then frees it. */
+int sb_server_remove_gather(sb_server* s, int gather_id);
+/* hbm_used: bytes held by running requests and resident gathers. */
 int sb_server_stats(const sb_server* s, uint64_t* done, uint64_t* launches, size_t* hbm_used);
 /* Stops the workers; queued requests that have not started fail. */
 void sb_server_destroy(sb_server* s);

@@ -69,7 +69,8 @@ class Request(C.Structure):
     _fields_ = [("method", C.c_int), ("ncdp", C.c_int), ("ns", C.c_int), ("dt_us", C.c_int),
                 ("cdp_off", C.POINTER(C.c_int32)), ("geom", C.c_void_p), ("samples", C.POINTER(C.c_float)),
                 ("nbr_off", C.POINTER(C.c_int32)), ("nbr", C.POINTER(C.c_int32)), ("cmp", CmpParams),
-                ("crs", CrsParams), ("store", C.c_int)]
+                ("crs", CrsParams), ("store", C.c_int), ("gather_id", C.c_int), ("cdp_begin", C.c_int),
+                ("cdp_end", C.c_int)]
 
 
 class RequestStats(C.Structure):
@@ -85,9 +86,10 @@ EXPORTS = [
     "sb_su_samples", "sb_su_geom", "sb_su_nbr_offsets", "sb_su_nbr", "sb_su_write_section", "sb_su_write_traces",
     "sb_crs_neighbors", "sb_keep_trace", "sb_ctx_create", "sb_ctx_destroy", "sb_ctx_memory", "sb_ctx_sync",
     "sb_gather_create", "sb_gather_destroy", "sb_gather_info", "sb_result_create", "sb_result_destroy",
-    "sb_result_download", "sb_cmp_run", "sb_crs_run", "sb_crs_de_run", "sb_cmp", "sb_crs", "sb_crs_de",
+    "sb_result_download", "sb_cmp_run", "sb_crs_run", "sb_crs_de_run", "sb_cmp_run_range", "sb_crs_run_range",
+    "sb_crs_de_run_range", "sb_cmp", "sb_crs", "sb_crs_de",
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
-    "sb_server_wait", "sb_server_stats", "sb_server_destroy",
+    "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
 ]
 
 
@@ -136,6 +138,9 @@ def lib():
         L.sb_cmp_run.argtypes = [P, P, C.POINTER(CmpParams), P]
         L.sb_crs_run.argtypes = [P, P, C.POINTER(CrsParams), P]
         L.sb_crs_de_run.argtypes = [P, P, C.POINTER(DeParams), P]
+        L.sb_cmp_run_range.argtypes = [P, P, C.POINTER(CmpParams), C.c_int, C.c_int, P]
+        L.sb_crs_run_range.argtypes = [P, P, C.POINTER(CrsParams), C.c_int, C.c_int, P]
+        L.sb_crs_de_run_range.argtypes = [P, P, C.POINTER(DeParams), C.c_int, C.c_int, P]
         L.sb_cmp.argtypes = [P, P, C.POINTER(CmpParams), C.POINTER(HostResult)]
         L.sb_crs.argtypes = [P, P, C.POINTER(CrsParams), C.POINTER(HostResult)]
         L.sb_crs_de.argtypes = [P, P, C.POINTER(DeParams), C.POINTER(HostResult)]
@@ -148,6 +153,9 @@ def lib():
         L.sb_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(P)]
         L.sb_server_submit.argtypes = [P, C.POINTER(Request), C.POINTER(C.c_uint64)]
         L.sb_server_wait.argtypes = [P, C.c_uint64, C.c_double, C.POINTER(HostResult), C.POINTER(RequestStats)]
+        L.sb_server_add_gather.argtypes = [P, C.c_int, i32p, C.c_int, C.c_int, P, C.POINTER(C.c_float), C.c_int,
+                                           i32p, i32p, C.POINTER(C.c_int)]
+        L.sb_server_remove_gather.argtypes = [P, C.c_int]
         L.sb_server_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_size_t)]
         L.sb_server_destroy.argtypes = [P]
         L.sb_server_destroy.restype = None
@@ -343,21 +351,23 @@ def de_params(a0, a1, b0, b1, c0, c1, tau, pop=32, ngen=30, F=0.5, CR=0.9, seed=
     return DeParams(a0, a1, b0, b1, c0, c1, tau, pop, ngen, F, CR, seed, mode)
 
 
-def run(ctx, gather, params, result):
-    """Queues one search (CMP, CRS or CRS-DE by params type) into result."""
+def run(ctx, gather, params, result, cdp_range=None):
+    """Queues one search (CMP, CRS or CRS-DE by params type) into result,
+    over all CDPs or over cdp_range = (begin, end)."""
     L = lib()
-    if isinstance(params, CmpParams):
-        check(L.sb_cmp_run(ctx.h, gather.h, C.byref(params), result.h))
-    elif isinstance(params, CrsParams):
-        check(L.sb_crs_run(ctx.h, gather.h, C.byref(params), result.h))
+    kind = "cmp" if isinstance(params, CmpParams) else "crs" if isinstance(params, CrsParams) else "crs_de"
+    if cdp_range is None:
+        check(getattr(L, "sb_%s_run" % kind)(ctx.h, gather.h, C.byref(params), result.h))
     else:
-        check(L.sb_crs_de_run(ctx.h, gather.h, C.byref(params), result.h))
+        check(getattr(L, "sb_%s_run_range" % kind)(ctx.h, gather.h, C.byref(params), int(cdp_range[0]),
+                                                   int(cdp_range[1]), result.h))
 
 
-def search(ctx, gather, params):
-    r = DeviceResult(ctx, gather.ncdp, gather.ns)
+def search(ctx, gather, params, cdp_range=None):
+    n = gather.ncdp if cdp_range is None else cdp_range[1] - cdp_range[0]
+    r = DeviceResult(ctx, n, gather.ns)
     try:
-        run(ctx, gather, params, r)
+        run(ctx, gather, params, r, cdp_range)
         return r.download()
     finally:
         r.close()
@@ -377,6 +387,7 @@ class Server:
         self.h = C.c_void_p()
         check(lib().sb_server_create(device, hbm_budget, max_batch, C.byref(self.h)))
         self._keep = {}
+        self._gathers = {}
 
     def submit(self, cdp_off, geom, samples, dt_us, params, store=STORE_F32, nbr_off=None, nbr=None):
         off = _i32(cdp_off)
@@ -398,6 +409,39 @@ class Server:
         t = C.c_uint64()
         check(lib().sb_server_submit(self.h, C.byref(req), C.byref(t)))
         self._keep[t.value] = (keep, req.ncdp, req.ns)
+        return t.value
+
+    def add_gather(self, cdp_off, geom, samples, dt_us, store=STORE_F32, nbr_off=None, nbr=None):
+        """Uploads a binned line once; returns its id for submit_range()."""
+        off = _i32(cdp_off)
+        g = np.ascontiguousarray(geom, GEOM_DTYPE)
+        s = np.ascontiguousarray(samples, np.float32)
+        no = nb = None
+        if nbr_off is not None:
+            no, nb = _i32(nbr_off), _i32(nbr if len(nbr) else [0])
+        gid = C.c_int()
+        check(lib().sb_server_add_gather(self.h, len(off) - 1, _ptr(off, C.c_int32), s.shape[1], dt_us,
+                                         g.ctypes.data, _ptr(s, C.c_float), store,
+                                         None if no is None else _ptr(no, C.c_int32),
+                                         None if nb is None else _ptr(nb, C.c_int32), C.byref(gid)))
+        self._gathers[gid.value] = s.shape[1]
+        return gid.value
+
+    def remove_gather(self, gid):
+        check(lib().sb_server_remove_gather(self.h, gid))
+        self._gathers.pop(gid, None)
+
+    def submit_range(self, gid, cdp_begin, cdp_end, params):
+        """Queues a search of CDPs [cdp_begin, cdp_end) of a resident gather."""
+        req = Request()
+        req.gather_id, req.cdp_begin, req.cdp_end = gid, cdp_begin, cdp_end
+        if isinstance(params, CmpParams):
+            req.method, req.cmp = METHOD_CMP, params
+        else:
+            req.method, req.crs = METHOD_CRS, params
+        t = C.c_uint64()
+        check(lib().sb_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._keep[t.value] = ([], cdp_end - cdp_begin, self._gathers[gid])
         return t.value
 
     def wait(self, ticket, timeout_ms=-1.0):

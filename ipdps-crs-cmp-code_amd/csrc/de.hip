@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_crs_de(SearchArgs a, DeArgs d, LD ld) {
   const float bc = __shfl(x[2], seg_base + bi, 64);
   const float bs = __shfl(stk, seg_base + bi, 64);
   if (member == 0 && t0 < a.ns) {
-    const int64_t o = (int64_t)cdp * a.ns + t0;
+    const int64_t o = (int64_t)(cdp - a.cdp_base) * a.ns + t0;
     a.out_index[o] = -1;
     a.out_a[o] = ba;
     a.out_b[o] = bb;

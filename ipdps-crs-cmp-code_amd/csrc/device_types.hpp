@@ -51,6 +51,7 @@ struct SearchArgs {
   const float* h_ref32;    // exact CMP h per row (or null)
   const int32_t* cdp_off;  // [ncdp+1] rows of each CDP
   int ncdp, ns, tau, w;
+  int cdp_base, nout;      // outputs cover CDPs [cdp_base, cdp_base + nout)
   int method;              // SB_METHOD_*
   // grid: CMP uses c only.  Float values for the fast search.
   float fa0, finc_a, fb0, finc_b, fc0, finc_c;
@@ -61,7 +62,7 @@ struct SearchArgs {
   float dt_f, idt_f;                             // CMP float dt, idt
   double dt_d, idt_d;                            // CRS double dt, idt
   // outputs
-  unsigned long long* keys;  // [ncdp][ns] packed best (sem bits << 32 | ~p)
+  unsigned long long* keys;  // [nout][ns] packed best (sem bits << 32 | ~p)
   int32_t* out_index;
   float *out_a, *out_b, *out_c, *out_coher, *out_stack;
   // work

@@ -343,7 +343,7 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
   if (wi.cdp < 0) return;
   const int t0 = wi.tile * 64 + (int)threadIdx.x;
   if (t0 >= a.ns) return;
-  const int64_t o = (int64_t)wi.cdp * a.ns + t0;
+  const int64_t o = (int64_t)(wi.cdp - a.cdp_base) * a.ns + t0;
   if (a.method == SB_METHOD_CMP) {
     float max_sem = 0.f, best_stk = 0.f;
     int max_c = -1;
@@ -385,10 +385,10 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
 // in the reference's precision and order; writes the output arrays.
 template <class LD>
 __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
-  const int64_t n = (int64_t)a.ncdp * a.ns;
+  const int64_t n = (int64_t)a.nout * a.ns;
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= n) return;
-  const int cdp = (int)(o / a.ns), t0 = (int)(o % a.ns);
+  const int cdp = a.cdp_base + (int)(o / a.ns), t0 = (int)(o % a.ns);
   const unsigned long long key = a.keys[o];
   const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
   if (a.method == SB_METHOD_CMP) {
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
         m = v > m ? v : m;
       }
       const int t0 = wi.tile * 64 * R + r * 64 + lane;
-      if (t0 < a.ns && m) atomicMax(&a.keys[(int64_t)cdp * a.ns + t0], m);
+      if (t0 < a.ns && m) atomicMax(&a.keys[(int64_t)(cdp - a.cdp_base) * a.ns + t0], m);
     }
   }
 }
@@ -665,7 +665,7 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 
 template <class LD>
 static hipError_t finalize_with(const SearchArgs& a, const LD& ld, hipStream_t st) {
-  const int64_t n = (int64_t)a.ncdp * a.ns;
+  const int64_t n = (int64_t)a.nout * a.ns;
   hipLaunchKernelGGL(k_finalize<LD>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
   return SB_CHECK_LAUNCH();
 }
@@ -677,7 +677,7 @@ static hipError_t exact_with(const SearchArgs& a, const LD& ld, int nblocks, hip
 }
 
 hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st) {
-  if ((int64_t)a.ncdp * a.ns <= 0) return hipSuccess;
+  if ((int64_t)a.nout * a.ns <= 0) return hipSuccess;
   if (store == SB_STORE_F32) return finalize_with(a, LoadF32{(const float*)a.samples, a.stride}, st);
   if (store == SB_STORE_F16) return finalize_with(a, LoadF16{(const __half*)a.samples, a.scale, a.stride}, st);
   return finalize_with(a, LoadBF16{(const uint16_t*)a.samples, a.scale, a.stride}, st);

@@ -38,20 +38,54 @@ int hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #call); \
   } while (0)
 
+// Caching device allocator: freed blocks are kept per ctx and reused for
+// requests of up to twice their size, so steady-state searches and serving
+// do no hipMalloc/hipFree (hipFree synchronises the device).  Cached bytes
+// count against the budget and are released first when it runs short.
+static size_t round_block(size_t bytes) {
+  if (bytes <= (1u << 20)) {
+    size_t r = 256;
+    while (r < bytes) r <<= 1;
+    return r;
+  }
+  return (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+}
+
+static void release_cache(sb_ctx* ctx) {
+  for (auto& kv : ctx->cache) {
+    (void)hipFree(kv.second);
+    ctx->used -= kv.first;
+  }
+  ctx->cache.clear();
+}
+
 int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
   *p = nullptr;
   if (bytes == 0) return SB_OK;
+  const size_t want = round_block(bytes);
   std::lock_guard<std::mutex> lk(ctx->mem_mu);
-  if (ctx->used + bytes > ctx->budget)
-    return fail(SB_ERR_BUDGET, "HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
+  auto it = ctx->cache.lower_bound(want);
+  if (it != ctx->cache.end() && it->first <= 2 * want) {
+    *p = it->second;
+    ctx->sizes[*p] = it->first;
+    ctx->cache.erase(it);
+    return SB_OK;
+  }
+  if (ctx->used + want > ctx->budget) release_cache(ctx);
+  if (ctx->used + want > ctx->budget)
+    return fail(SB_ERR_BUDGET, "HBM budget exceeded: need " + std::to_string(want) + " B, " +
                                    std::to_string(ctx->budget - ctx->used) + " B left");
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = hipMalloc(p, want);
+  if (e != hipSuccess && !ctx->cache.empty()) {
+    release_cache(ctx);
+    e = hipMalloc(p, want);
+  }
   if (e != hipSuccess) {
     *p = nullptr;
     return hip_fail(e, "hipMalloc");
   }
-  ctx->used += bytes;
-  ctx->sizes[*p] = bytes;
+  ctx->used += want;
+  ctx->sizes[*p] = want;
   return SB_OK;
 }
 
@@ -59,11 +93,30 @@ void ctx_free(sb_ctx* ctx, void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(ctx->mem_mu);
   auto it = ctx->sizes.find(p);
-  if (it != ctx->sizes.end()) {
-    ctx->used -= it->second;
-    ctx->sizes.erase(it);
+  if (it == ctx->sizes.end()) return;
+  ctx->cache.insert({it->second, p});
+  ctx->sizes.erase(it);
+}
+
+void ctx_trim(sb_ctx* ctx, size_t keep) {
+  std::lock_guard<std::mutex> lk(ctx->mem_mu);
+  size_t cached = 0;
+  for (auto& kv : ctx->cache) cached += kv.first;
+  while (cached > keep && !ctx->cache.empty()) {
+    auto it = std::prev(ctx->cache.end());  // largest first
+    (void)hipFree(it->second);
+    ctx->used -= it->first;
+    cached -= it->first;
+    ctx->cache.erase(it);
   }
-  (void)hipFree(p);
+}
+
+void ctx_release_all(sb_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(ctx->mem_mu);
+  release_cache(ctx);
+  for (auto& kv : ctx->sizes) (void)hipFree(kv.first);
+  ctx->sizes.clear();
+  ctx->used = 0;
 }
 
 }  // namespace sb
@@ -113,8 +166,7 @@ extern "C" void sb_ctx_destroy(sb_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  ctx_free(ctx, ctx->work_buf);
-  ctx_free(ctx, ctx->queue_buf);
+  ctx_release_all(ctx);
   for (auto& ev : ctx->ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -139,7 +191,9 @@ extern "C" void sb_gather_destroy(sb_gather* g) {
   if (!g) return;
   sb_ctx* c = g->ctx;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  // Searches from other contexts may still read the gather; its blocks go
+  // back to the cache for reuse, so wait for the whole device.
+  (void)hipDeviceSynchronize();
   for (void* p : {(void*)g->d_samples, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
                   (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
                   (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p})
@@ -358,16 +412,16 @@ namespace sb {
 // Work table: CDPs in descending cost, dealt round-robin over the 8 XCDs
 // (consecutive workgroups land on consecutive XCDs), so every block of a
 // CDP shares one L2 and heavy CDPs start first on every XCD.
-int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int ntiles, int npar, int pchunk, int* nblocks,
-               const std::string& key) {
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
+               int* nblocks, const std::string& key) {
   if (!key.empty() && key == ctx->work_key) {
     *nblocks = ctx->work_n;
     return SB_OK;
   }
   ctx->work_key.clear();
-  const int ncdp = (int)cost.size();
+  const int ncdp = ce - cb;
   std::vector<int> order(ncdp);
-  std::iota(order.begin(), order.end(), 0);
+  std::iota(order.begin(), order.end(), cb);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
   constexpr int kXcd = 8;
@@ -414,15 +468,17 @@ namespace {
 
 // Identifies a work table: gather object + its generation, method, kernel
 // family and tiling, so repeated calls on one gather reuse the uploaded table.
-std::string work_key(const sb_gather* g, int method, int family, int ntiles, int npar, int pchunk) {
-  char buf[160];
-  std::snprintf(buf, sizeof buf, "%p/%llu/%d/%d/%d/%d/%d", (const void*)g, (unsigned long long)g->serial, method,
-                family, ntiles, npar, pchunk);
+std::string work_key(const sb_gather* g, int method, int family, int ntiles, int npar, int pchunk, int cb, int ce) {
+  char buf[192];
+  std::snprintf(buf, sizeof buf, "%p/%llu/%d/%d/%d/%d/%d/%d/%d", (const void*)g, (unsigned long long)g->serial,
+                method, family, ntiles, npar, pchunk, cb, ce);
   return buf;
 }
 
-void base_args(const sb_gather* g, sb_result* r, SearchArgs& a) {
+void base_args(const sb_gather* g, sb_result* r, int cb, SearchArgs& a) {
   std::memset(&a, 0, sizeof a);
+  a.cdp_base = cb;
+  a.nout = r->ncdp;
   a.samples = g->d_samples;
   a.scale = g->d_scale;
   a.stride = g->stride;
@@ -439,9 +495,13 @@ void base_args(const sb_gather* g, sb_result* r, SearchArgs& a) {
   a.out_stack = r->stack;
 }
 
-int check_result(const sb_gather* g, const sb_result* r) {
-  if (r->ncdp != g->ncdp || r->ns != g->ns) return fail(SB_ERR_INVALID, "result size does not match gather");
-  if (r->ctx != g->ctx) return fail(SB_ERR_INVALID, "result and gather belong to different contexts");
+// A result covers CDPs [cb, ce) of the gather; gather and result may belong
+// to different contexts of the same device (a resident gather is shared).
+int check_range(const sb_ctx* ctx, const sb_gather* g, const sb_result* r, int cb, int ce) {
+  if (cb < 0 || ce > g->ncdp || cb >= ce) return fail(SB_ERR_INVALID, "bad CDP range");
+  if (r->ncdp != ce - cb || r->ns != g->ns) return fail(SB_ERR_INVALID, "result size does not match the CDP range");
+  if (r->ctx != ctx || g->ctx->device != ctx->device)
+    return fail(SB_ERR_INVALID, "result must belong to the ctx, gather to the same device");
   return SB_OK;
 }
 
@@ -457,18 +517,15 @@ int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int cc) {
 }
 
 int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int mode, double params_per_tuple,
-               const std::vector<int64_t>& S) {
+               const std::vector<int64_t>& S, int cb, int ce) {
   HIPR(hipSetDevice(ctx->device));
   std::lock_guard<std::mutex> lk(ctx->run_mu);
   int rc;
   int nblocks = 0;
   const int store = g->store;
-  std::vector<int64_t> cost(g->ncdp);
+  const std::vector<int64_t>& cost = S;
   double tuples = 0;
-  for (int i = 0; i < g->ncdp; i++) {
-    cost[i] = S[i];
-    tuples += (double)S[i] * g->ns * params_per_tuple;
-  }
+  for (int i = cb; i < ce; i++) tuples += (double)S[i] * g->ns * params_per_tuple;
   int variant = -1;
   if (mode == SB_MODE_FAST) {
     if (!fast_supported(a.w)) return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
@@ -476,17 +533,17 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
     const FastVariant v = kFastVariants[variant];
     const int tile = 64 * v.R;
     const int ntiles = (g->ns + tile - 1) / tile;
-    const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override : choose_pchunk(ctx, g->ncdp, ntiles, a.npar, v.CC);
-    const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk);
-    if ((rc = build_work(ctx, cost, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+    const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.CC);
+    const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
+    if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
   } else {
     const int ntiles = (g->ns + 63) / 64;
-    const std::string key = work_key(g, a.method, 1, ntiles, 1, 1);
-    if ((rc = build_work(ctx, cost, ntiles, 1, 1, &nblocks, key))) return rc;
+    const std::string key = work_key(g, a.method, 1, ntiles, 1, 1, cb, ce);
+    if ((rc = build_work(ctx, cost, cb, ce, ntiles, 1, 1, &nblocks, key))) return rc;
   }
   a.work = (const WorkItem*)ctx->work_buf;
   a.nwork = nblocks;
-  const int64_t nout = (int64_t)g->ncdp * g->ns;
+  const int64_t nout = (int64_t)(ce - cb) * g->ns;
   HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
   hipError_t e = hipSuccess;
   if (mode == SB_MODE_FAST) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
@@ -514,13 +571,19 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
 }  // namespace
 
 extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_result* r) {
+  if (!g) return fail(SB_ERR_INVALID, "sb_cmp_run: null gather");
+  return sb_cmp_run_range(ctx, g, p, 0, g->ncdp, r);
+}
+
+extern "C" int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cb, int ce,
+                                sb_result* r) {
   if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_cmp_run: null argument");
   int rc;
-  if ((rc = check_result(g, r))) return rc;
+  if ((rc = check_range(ctx, g, r, cb, ce))) return rc;
   if (p->nc <= 0) return fail(SB_ERR_INVALID, "nc must be positive");
   if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
   SearchArgs a;
-  base_args(g, r, a);
+  base_args(g, r, cb, a);
   a.method = SB_METHOD_CMP;
   a.slot_off = g->d_cdp_off;
   a.slots = g->d_cmp_slots;
@@ -551,19 +614,25 @@ extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* 
   // The fast kernel assumes t0^2 + C h^2 >= 0; a negative C range (NaN
   // traveltimes, all invalid in the reference) runs in reference arithmetic.
   const int mode = (c0 < 0.f || c1 < 0.f) ? SB_MODE_EXACT : p->mode;
-  return run_search(ctx, g, r, a, mode, (double)p->nc, S);
+  return run_search(ctx, g, r, a, mode, (double)p->nc, S, cb, ce);
 }
 
 extern "C" int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_result* r) {
+  if (!g) return fail(SB_ERR_INVALID, "sb_crs_run: null gather");
+  return sb_crs_run_range(ctx, g, p, 0, g->ncdp, r);
+}
+
+extern "C" int sb_crs_run_range(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, int cb, int ce,
+                                sb_result* r) {
   if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_crs_run: null argument");
   int rc;
-  if ((rc = check_result(g, r))) return rc;
+  if ((rc = check_range(ctx, g, r, cb, ce))) return rc;
   if (!g->has_crs) return fail(SB_ERR_STATE, "gather was created without neighbour lists");
   if (p->na <= 0 || p->nb <= 0 || p->nc <= 0) return fail(SB_ERR_INVALID, "na, nb, nc must be positive");
   if ((int64_t)p->na * p->nb * p->nc > INT32_MAX / 2) return fail(SB_ERR_INVALID, "parameter grid too large");
   if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
   SearchArgs a;
-  base_args(g, r, a);
+  base_args(g, r, cb, a);
   a.method = SB_METHOD_CRS;
   a.slot_off = g->d_crs_slot_off;
   a.slots = g->d_crs_slots;
@@ -601,20 +670,26 @@ extern "C" int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* 
   a.idt_d = idt;
   std::vector<int64_t> S(g->ncdp);
   for (int i = 0; i < g->ncdp; i++) S[i] = g->crs_slot_off[i + 1] - g->crs_slot_off[i];
-  return run_search(ctx, g, r, a, p->mode, (double)a.npar, S);
+  return run_search(ctx, g, r, a, p->mode, (double)a.npar, S, cb, ce);
 }
 
 extern "C" int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_result* r) {
+  if (!g) return fail(SB_ERR_INVALID, "sb_crs_de_run: null gather");
+  return sb_crs_de_run_range(ctx, g, p, 0, g->ncdp, r);
+}
+
+extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, int cb, int ce,
+                                   sb_result* r) {
   if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_crs_de_run: null argument");
   int rc;
-  if ((rc = check_result(g, r))) return rc;
+  if ((rc = check_range(ctx, g, r, cb, ce))) return rc;
   if (!g->has_crs) return fail(SB_ERR_STATE, "gather was created without neighbour lists");
   if (p->pop < 4 || p->pop > 64) return fail(SB_ERR_INVALID, "population must be in [4, 64]");
   if (p->ngen < 0) return fail(SB_ERR_INVALID, "ngen must be >= 0");
   if (!(p->CR >= 0 && p->CR <= 1) || !(p->F >= 0 && p->F <= 2)) return fail(SB_ERR_INVALID, "bad F or CR");
   if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
   SearchArgs a;
-  base_args(g, r, a);
+  base_args(g, r, cb, a);
   a.method = SB_METHOD_CRS;
   a.slot_off = g->d_crs_slot_off;
   a.slots = g->d_crs_slots;
@@ -658,10 +733,11 @@ extern "C" int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params
   double tuples = 0;
   for (int i = 0; i < g->ncdp; i++) {
     cost[i] = g->crs_slot_off[i + 1] - g->crs_slot_off[i];
-    tuples += (double)cost[i] * g->ns * (double)p->pop * (p->ngen + 1);
+    if (i >= cb && i < ce) tuples += (double)cost[i] * g->ns * (double)p->pop * (p->ngen + 1);
   }
   int nblocks = 0;
-  if ((rc = build_work(ctx, cost, ntiles, 1, 1, &nblocks, work_key(g, SB_METHOD_CRS, 2, ntiles, 1, 1)))) return rc;
+  if ((rc = build_work(ctx, cost, cb, ce, ntiles, 1, 1, &nblocks, work_key(g, SB_METHOD_CRS, 2, ntiles, 1, 1, cb, ce))))
+    return rc;
   a.work = (const WorkItem*)ctx->work_buf;
   a.nwork = nblocks;
   HIPR(hipEventRecord(ctx->ev[0], ctx->stream));

@@ -16,7 +16,8 @@ struct sb_ctx {
   hipStream_t stream = nullptr;
   size_t budget = 0, used = 0;
   std::mutex mem_mu, run_mu;
-  std::map<void*, size_t> sizes;
+  std::map<void*, size_t> sizes;           // live blocks
+  std::multimap<size_t, void*> cache;      // freed blocks kept for reuse
   hipEvent_t ev[4] = {};
   sb_timing last{};
   bool timing_pending = false;
@@ -67,7 +68,10 @@ struct sb_result {
 namespace sb {
 int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p);
 void ctx_free(sb_ctx* ctx, void* p);
+void ctx_release_all(sb_ctx* ctx);
+// Frees cached blocks, largest first, until at most keep bytes stay cached.
+void ctx_trim(sb_ctx* ctx, size_t keep);
 int hip_fail(hipError_t e, const char* what);
-int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int ntiles, int npar, int pchunk, int* nblocks,
-               const std::string& key);
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
+               int* nblocks, const std::string& key);
 }  // namespace sb

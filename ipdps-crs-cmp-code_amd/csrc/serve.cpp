@@ -11,6 +11,11 @@
 //     (method, ns, dt, parameters, store) into one gather and one launch, up
 //     to max_batch CDPs, and splits the results back per request;
 //   - sb_server_wait blocks for one ticket and copies its results out.
+// Lines registered with sb_server_add_gather stay on the device; requests
+// that name one and a CDP range skip the upload, and the requests merged
+// into one batch run as range searches on the slot's stream into one result
+// buffer, downloaded once.  Device blocks come from each slot ctx's caching
+// allocator, so steady-state requests do no hipMalloc/hipFree.
 // A request's results do not depend on what it was batched with: every
 // evaluation's arithmetic is fixed (no shape-dependent contraction) and the
 // arg-max key is exact.
@@ -35,16 +40,31 @@ using namespace sb;
 namespace {
 
 constexpr int kSlots = 4;  // matches the 4 hardware queues a process gets by default
+// Slot 0 only takes small requests (at most kSmallEvals evaluations, a few ms
+// of GPU time), so interactive panels never wait behind a long search.
+constexpr double kSmallEvals = 2e10;
+// Device bytes each slot keeps cached between requests (outside the budget).
+constexpr size_t kSlotCache = size_t(512) << 20;
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// A line kept on the device for range requests.
+struct Resident {
+  sb_gather* g = nullptr;
+  size_t bytes = 0;
+  int users = 0;  // requests queued or running on it
+  bool removing = false;
+};
+
 struct Req {
   uint64_t ticket = 0;
   sb_request r{};
+  Resident* res = nullptr;  // resident gather, or null when the request carries traces
   int ntr = 0;
   size_t bytes = 0;  // device footprint estimate
+  double evals = 0;  // CDP x t0 x parameter x trace evaluations
   double t_submit = 0, t_admit = 0, t_done = 0;
   int status = -1;  // -1 pending/running, else final status
   bool running = false;
@@ -55,6 +75,7 @@ struct Req {
 // Same launch shape: requests with equal keys can share one gather and launch.
 bool same_shape(const sb_request& x, const sb_request& y) {
   if (x.method != y.method || x.ns != y.ns || x.dt_us != y.dt_us || x.store != y.store) return false;
+  if (x.gather_id != y.gather_id) return false;
   if (x.method == SB_METHOD_CMP)
     return x.cmp.c0 == y.cmp.c0 && x.cmp.c1 == y.cmp.c1 && x.cmp.nc == y.cmp.nc && x.cmp.tau == y.cmp.tau &&
            x.cmp.mode == y.cmp.mode;
@@ -80,6 +101,26 @@ size_t footprint(const sb_request& r, int ntr) {
   return b;
 }
 
+// Range request on a resident gather: only the results and the work table.
+size_t footprint_range(const sb_request& r) { return (size_t)(r.cdp_end - r.cdp_begin) * r.ns * 32 + (1 << 20); }
+
+double evaluations_range(const sb_request& r, const sb_gather* g) {
+  const int cb = r.cdp_begin, ce = r.cdp_end;
+  if (r.method == SB_METHOD_CMP) return (double)(g->cdp_off[ce] - g->cdp_off[cb]) * r.ns * r.cmp.nc;
+  return (double)(g->crs_slot_off[ce] - g->crs_slot_off[cb]) * r.ns * (double)r.crs.na * r.crs.nb * r.crs.nc;
+}
+
+double evaluations(const sb_request& r) {
+  double traces = 0;
+  if (r.method == SB_METHOD_CMP) {
+    traces = r.cdp_off[r.ncdp];
+    return traces * r.ns * r.cmp.nc;
+  }
+  for (int i = 0; i < r.ncdp; i++)
+    for (int q = r.nbr_off[i]; q < r.nbr_off[i + 1]; q++) traces += r.cdp_off[r.nbr[q] + 1] - r.cdp_off[r.nbr[q]];
+  return traces * r.ns * (double)r.crs.na * r.crs.nb * r.crs.nc;
+}
+
 }  // namespace
 
 struct sb_server {
@@ -95,12 +136,68 @@ struct sb_server {
   bool stop = false;
   std::vector<std::thread> workers;
   std::vector<sb_ctx*> ctx;
+  sb_ctx* res_ctx = nullptr;  // owns the resident gathers
+  std::map<int, Resident> resident;
+  int next_gid = 1;
 
   void worker(int slot);
   int process(int slot, std::vector<std::shared_ptr<Req>>& batch);
+  int process_range(int slot, std::vector<std::shared_ptr<Req>>& batch);
 };
 
+// Batch of range requests on one resident gather: one result buffer, one
+// range search per request into its rows, one download.
+int sb_server::process_range(int slot, std::vector<std::shared_ptr<Req>>& batch) {
+  sb_ctx* c = ctx[slot];
+  const sb_request& r0 = batch[0]->r;
+  const sb_gather* g = batch[0]->res->g;
+  int rows = 0;
+  for (auto& q : batch) rows += q->r.cdp_end - q->r.cdp_begin;
+  sb_result* res = nullptr;
+  int rc = sb_result_create(c, rows, r0.ns, &res);
+  if (rc) return rc;
+  int o = 0;
+  for (auto& q : batch) {
+    const int cb = q->r.cdp_begin, ce = q->r.cdp_end;
+    sb_result view = *res;  // rows [o, o + ce - cb) of res
+    const size_t shift = (size_t)o * r0.ns;
+    view.ncdp = ce - cb;
+    view.index += shift;
+    view.a += shift;
+    view.b += shift;
+    view.c += shift;
+    view.coher += shift;
+    view.stack += shift;
+    view.keys += shift;
+    rc = r0.method == SB_METHOD_CMP ? sb_cmp_run_range(c, g, &q->r.cmp, cb, ce, &view)
+                                    : sb_crs_run_range(c, g, &q->r.crs, cb, ce, &view);
+    if (rc) break;
+    o += ce - cb;
+  }
+  if (!rc) {
+    const size_t n = (size_t)rows * r0.ns;
+    std::vector<int32_t> idx(n);
+    std::vector<float> a(n), b(n), cc(n), coh(n), stk(n);
+    sb_host_result h{idx.data(), a.data(), b.data(), cc.data(), coh.data(), stk.data()};
+    rc = sb_result_download(c, res, &h);
+    size_t k = 0;
+    for (auto& q : batch) {
+      const size_t m = (size_t)(q->r.cdp_end - q->r.cdp_begin) * r0.ns;
+      q->index.assign(idx.begin() + k, idx.begin() + k + m);
+      q->a.assign(a.begin() + k, a.begin() + k + m);
+      q->b.assign(b.begin() + k, b.begin() + k + m);
+      q->c.assign(cc.begin() + k, cc.begin() + k + m);
+      q->coher.assign(coh.begin() + k, coh.begin() + k + m);
+      q->stack.assign(stk.begin() + k, stk.begin() + k + m);
+      k += m;
+    }
+  }
+  sb_result_destroy(res);
+  return rc;
+}
+
 int sb_server::process(int slot, std::vector<std::shared_ptr<Req>>& batch) {
+  if (batch[0]->res) return process_range(slot, batch);
   sb_ctx* c = ctx[slot];
   const sb_request& r0 = batch[0]->r;
   // Concatenate the batch into one binned gather (CRS neighbour indices shifted).
@@ -185,6 +282,7 @@ void sb_server::worker(int slot) {
       // Oldest request that fits the free budget; a request alone larger
       // than the whole budget is rejected.
       for (auto& q : pending) {
+        if (slot == 0 && q->evals > kSmallEvals) continue;
         if (q->bytes > budget || reserved + q->bytes <= budget) {
           head = q;
           return true;
@@ -200,6 +298,7 @@ void sb_server::worker(int slot) {
     if (head->bytes > budget) {
       head->status = SB_ERR_BUDGET;
       head->t_done = now_ms(t0);
+      if (head->res) head->res->users--;
       done++;
       cv_done.notify_all();
       continue;
@@ -209,7 +308,8 @@ void sb_server::worker(int slot) {
     int cdps = head->r.ncdp;
     for (auto it = pending.begin(); it != pending.end() && cdps < max_batch;) {
       auto& q = *it;
-      if (same_shape(q->r, head->r) && cdps + q->r.ncdp <= max_batch && reserved + bytes + q->bytes <= budget) {
+      if (same_shape(q->r, head->r) && cdps + q->r.ncdp <= max_batch && reserved + bytes + q->bytes <= budget &&
+          (slot != 0 || q->evals <= kSmallEvals)) {
         batch.push_back(q);
         bytes += q->bytes;
         cdps += q->r.ncdp;
@@ -227,6 +327,7 @@ void sb_server::worker(int slot) {
     launches++;
     lk.unlock();
     const int rc = process(slot, batch);
+    ctx_trim(ctx[slot], kSlotCache);
     lk.lock();
     reserved -= bytes;
     const double td = now_ms(t0);
@@ -234,6 +335,7 @@ void sb_server::worker(int slot) {
       q->status = rc;
       q->t_done = td;
       q->running = false;
+      if (q->res) q->res->users--;
       done++;
     }
     cv_done.notify_all();
@@ -256,6 +358,13 @@ extern "C" int sb_server_create(int device, size_t hbm_budget, int max_batch, sb
     }
     s->ctx.push_back(c);
   }
+  {
+    int rc = sb_ctx_create(device, hbm_budget, &s->res_ctx);
+    if (rc) {
+      for (auto* x : s->ctx) sb_ctx_destroy(x);
+      return rc;
+    }
+  }
   // Every slot ctx may use the whole budget; the server keeps the sum within it.
   size_t b = 0;
   sb_ctx_memory(s->ctx[0], nullptr, &b);
@@ -268,8 +377,43 @@ extern "C" int sb_server_create(int device, size_t hbm_budget, int max_batch, sb
 extern "C" int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* ticket) {
   if (!s || !req || !ticket) return fail(SB_ERR_INVALID, "sb_server_submit: bad arguments");
   const sb_request& r = *req;
-  if ((r.method != SB_METHOD_CMP && r.method != SB_METHOD_CRS) || r.ncdp <= 0 || r.ns <= 0 || r.dt_us <= 0 ||
-      !r.cdp_off || !r.geom || !r.samples || (r.method == SB_METHOD_CRS && (!r.nbr_off || !r.nbr)))
+  if (r.method != SB_METHOD_CMP && r.method != SB_METHOD_CRS)
+    return fail(SB_ERR_INVALID, "sb_server_submit: bad method");
+  if (r.gather_id != 0) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->resident.find(r.gather_id);
+    if (it == s->resident.end() || it->second.removing)
+      return fail(SB_ERR_INVALID, "sb_server_submit: unknown resident gather");
+    Resident& R = it->second;
+    if (r.cdp_begin < 0 || r.cdp_end > R.g->ncdp || r.cdp_begin >= r.cdp_end)
+      return fail(SB_ERR_INVALID, "sb_server_submit: bad CDP range");
+    if (r.method == SB_METHOD_CRS && !R.g->has_crs)
+      return fail(SB_ERR_STATE, "sb_server_submit: gather has no neighbour lists");
+    if (s->stop) return fail(SB_ERR_STATE, "server is shutting down");
+    auto q = std::make_shared<Req>();
+    q->r = r;
+    q->r.ncdp = r.cdp_end - r.cdp_begin;
+    q->r.ns = R.g->ns;
+    q->r.dt_us = R.g->dt_us;
+    q->r.store = R.g->store;
+    q->r.cdp_off = nullptr;
+    q->r.geom = nullptr;
+    q->r.samples = nullptr;
+    q->r.nbr_off = q->r.nbr = nullptr;
+    q->res = &R;
+    q->bytes = footprint_range(q->r);
+    q->evals = evaluations_range(q->r, R.g);
+    R.users++;
+    q->ticket = s->next_ticket++;
+    q->t_submit = now_ms(s->t0);
+    s->pending.push_back(q);
+    s->all[q->ticket] = q;
+    *ticket = q->ticket;
+    s->cv_work.notify_all();
+    return SB_OK;
+  }
+  if (r.ncdp <= 0 || r.ns <= 0 || r.dt_us <= 0 || !r.cdp_off || !r.geom || !r.samples ||
+      (r.method == SB_METHOD_CRS && (!r.nbr_off || !r.nbr)))
     return fail(SB_ERR_INVALID, "sb_server_submit: bad request");
   if (r.cdp_off[0] != 0) return fail(SB_ERR_INVALID, "cdp_off[0] must be 0");
   for (int i = 0; i < r.ncdp; i++)
@@ -278,6 +422,7 @@ extern "C" int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* t
   q->r = r;
   q->ntr = r.cdp_off[r.ncdp];
   q->bytes = footprint(r, q->ntr);
+  q->evals = evaluations(r);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return fail(SB_ERR_STATE, "server is shutting down");
   q->ticket = s->next_ticket++;
@@ -285,7 +430,7 @@ extern "C" int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* t
   s->pending.push_back(q);
   s->all[q->ticket] = q;
   *ticket = q->ticket;
-  s->cv_work.notify_one();
+  s->cv_work.notify_all();
   return SB_OK;
 }
 
@@ -317,6 +462,67 @@ extern "C" int sb_server_wait(sb_server* s, uint64_t ticket, double timeout_ms, 
   return SB_OK;
 }
 
+extern "C" int sb_server_add_gather(sb_server* s, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
+                                    const sb_trace_geom* geom, const float* samples, int store,
+                                    const int32_t* nbr_off, const int32_t* nbr, int* gather_id) {
+  if (!s || !gather_id || ncdp <= 0 || !cdp_off) return fail(SB_ERR_INVALID, "sb_server_add_gather: bad arguments");
+  *gather_id = 0;
+  sb_request est{};
+  est.method = nbr_off ? SB_METHOD_CRS : SB_METHOD_CMP;
+  est.ncdp = ncdp;
+  est.ns = ns;
+  est.store = store;
+  est.cdp_off = cdp_off;
+  est.nbr_off = nbr_off;
+  est.nbr = nbr;
+  if (cdp_off[0] != 0) return fail(SB_ERR_INVALID, "cdp_off[0] must be 0");
+  for (int i = 0; i < ncdp; i++)
+    if (cdp_off[i + 1] <= cdp_off[i]) return fail(SB_ERR_INVALID, "every CDP needs at least one trace");
+  if (nbr_off)
+    for (int i = 0; i < ncdp; i++)
+      for (int k = nbr_off[i]; k < nbr_off[i + 1]; k++)
+        if (nbr[k] < 0 || nbr[k] >= ncdp) return fail(SB_ERR_INVALID, "neighbour index out of range");
+  const size_t want = footprint(est, cdp_off[ncdp]);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(SB_ERR_STATE, "server is shutting down");
+    if (s->reserved + want > s->budget) return fail(SB_ERR_BUDGET, "resident gather does not fit the budget");
+    s->reserved += want;  // held while uploading
+  }
+  sb_gather* g = nullptr;
+  const int rc = sb_gather_create(s->res_ctx, ncdp, cdp_off, ns, dt_us, geom, samples, store, nbr_off, nbr, &g);
+  ctx_trim(s->res_ctx, 0);  // upload staging
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->reserved -= want;
+  if (rc) return rc;
+  const int id = s->next_gid++;
+  Resident& R = s->resident[id];
+  R.g = g;
+  R.bytes = g->bytes;
+  s->reserved += R.bytes;
+  *gather_id = id;
+  return SB_OK;
+}
+
+extern "C" int sb_server_remove_gather(sb_server* s, int gather_id) {
+  if (!s) return fail(SB_ERR_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->resident.find(gather_id);
+  if (it == s->resident.end() || it->second.removing) return fail(SB_ERR_INVALID, "unknown resident gather");
+  Resident& R = it->second;
+  R.removing = true;
+  s->cv_done.wait(lk, [&] { return R.users == 0; });
+  sb_gather* g = R.g;
+  const size_t bytes = R.bytes;
+  s->resident.erase(it);
+  lk.unlock();
+  sb_gather_destroy(g);  // waits for the device
+  lk.lock();
+  s->reserved -= bytes;
+  s->cv_work.notify_all();
+  return SB_OK;
+}
+
 extern "C" int sb_server_stats(const sb_server* cs, uint64_t* done, uint64_t* launches, size_t* hbm_used) {
   sb_server* s = const_cast<sb_server*>(cs);
   if (!s) return fail(SB_ERR_INVALID, "null server");
@@ -335,6 +541,7 @@ extern "C" void sb_server_destroy(sb_server* s) {
     // queued requests that never started fail
     for (auto& q : s->pending) {
       q->status = SB_ERR_STATE;
+      if (q->res) q->res->users--;
       s->done++;
     }
     s->pending.clear();
@@ -342,6 +549,9 @@ extern "C" void sb_server_destroy(sb_server* s) {
   s->cv_work.notify_all();
   s->cv_done.notify_all();
   for (auto& t : s->workers) t.join();
+  for (auto& kv : s->resident) sb_gather_destroy(kv.second.g);
+  s->resident.clear();
   for (auto* c : s->ctx) sb_ctx_destroy(c);
+  sb_ctx_destroy(s->res_ctx);
   delete s;
 }

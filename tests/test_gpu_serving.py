@@ -94,3 +94,94 @@ def test_admission_by_budget(sb):
     for o, _ in res[1:]:
         assert np.array_equal(o["coher"], first["coher"])
     assert out is None and "budget" in st["error"].lower()
+
+
+def _line(sb, kind, seed):
+    if kind == "cmp":
+        off, geom, s = sb.synth(0, 10, (24, 120), 701, 2000, seed, h_max=1500.0, h_jitter=5.0, nevents=5)
+        return dict(cdp_off=off, geom=geom, samples=s, dt_us=2000)
+    off, geom, s = sb.synth(1, 9, (40, 100), 601, 4000, seed, h_max=800.0, mid_half_width=20.0, nevents=4)
+    no, nb = sb.neighbors(geom[off[:-1]], 30)
+    return dict(cdp_off=off, geom=geom, samples=s, dt_us=4000, nbr_off=no, nbr=nb)
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+@pytest.mark.parametrize("kind", ["cmp", "crs", "crs_de"])
+def test_range_search_equals_rows_of_full_search(sb, gpu_ctx, kind, mode):
+    """sb_*_run_range over [cb, ce) gives exactly rows cb..ce-1 of the search
+    over the whole gather (the work table, keys and DE draws are per CDP)."""
+    print("case: range search %s %s" % (kind, mode), flush=True)
+    m = sb.MODE_FAST if mode == "fast" else sb.MODE_EXACT
+    L = _line(sb, "cmp" if kind == "cmp" else "crs", 11)
+    kw = {k: L[k] for k in ("nbr_off", "nbr") if k in L}
+    if kind == "cmp":
+        prm = sb.cmp_params(1.98e-7, 1.77e-6, 48, 0.002, mode=m)
+    elif kind == "crs":
+        prm = sb.crs_params(BOX[0], BOX[1], 4, BOX[2], BOX[3], 3, BOX[4], BOX[5], 5, 0.004, mode=m)
+    else:
+        prm = sb.de_params(*BOX, 0.004, pop=16, ngen=6, seed=5, mode=m)
+    g = sb.Gather(gpu_ctx, L["cdp_off"], L["geom"], L["samples"], L["dt_us"], **kw)
+    try:
+        full = sb.search(gpu_ctx, g, prm)
+        for cb, ce in ((0, 1), (3, 7), (g.ncdp - 2, g.ncdp), (0, g.ncdp)):
+            part = sb.search(gpu_ctx, g, prm, cdp_range=(cb, ce))
+            for k in ("index", "a", "b", "c", "coher", "stack"):
+                assert np.array_equal(part[k], full[k][cb:ce]), (cb, ce, k)
+        with pytest.raises(sb.SbError):
+            sb.search(gpu_ctx, g, prm, cdp_range=(2, 2))
+        with pytest.raises(sb.SbError):
+            sb.search(gpu_ctx, g, prm, cdp_range=(0, g.ncdp + 1))
+    finally:
+        g.close()
+
+
+def test_resident_gathers_alone_and_in_the_mix(sb, gpu_ctx):
+    """Range requests on lines kept resident by the server equal the range
+    searched alone, while they are merged among themselves and mixed with
+    requests that carry their traces."""
+    print("case: resident-gather serving mix", flush=True)
+    lines = {"cmp": _line(sb, "cmp", 21), "crs": _line(sb, "crs", 22)}
+    cprm = sb.cmp_params(1.98e-7, 1.77e-6, 64, 0.002)
+    rprm = sb.crs_params(BOX[0], BOX[1], 4, BOX[2], BOX[3], 4, BOX[4], BOX[5], 4, 0.004)
+    ref = {}
+    for kind, L in lines.items():
+        kw = {k: L[k] for k in ("nbr_off", "nbr") if k in L}
+        g = sb.Gather(gpu_ctx, L["cdp_off"], L["geom"], L["samples"], L["dt_us"], **kw)
+        try:
+            ref[kind] = sb.search(gpu_ctx, g, cprm if kind == "cmp" else rprm)
+        finally:
+            g.close()
+    rng = np.random.default_rng(3)
+    carried = make_requests(sb, 8, 9)
+    with sb.Server(0, max_batch=16) as srv:
+        gid = {k: srv.add_gather(L["cdp_off"], L["geom"], L["samples"], L["dt_us"],
+                                 **{q: L[q] for q in ("nbr_off", "nbr") if q in L}) for k, L in lines.items()}
+        used0 = srv.stats()["hbm_reserved"]
+        assert used0 > 0
+        jobs = []
+        for i in range(40):
+            kind = "cmp" if i % 3 else "crs"
+            n = lines[kind]["cdp_off"].size - 1
+            cb = int(rng.integers(0, n))
+            ce = int(rng.integers(cb + 1, min(n, cb + 4) + 1))
+            jobs.append((kind, cb, ce, srv.submit_range(gid[kind], cb, ce, cprm if kind == "cmp" else rprm)))
+            if i % 5 == 0:
+                r = carried[i // 5]
+                jobs.append(("carried", i // 5, None, srv.submit(*r["args"], **r["kw"])))
+        outs = [(j, srv.wait(j[3])) for j in jobs]
+        st = srv.stats()
+        with pytest.raises(sb.SbError):
+            srv.submit_range(gid["cmp"], 0, 99, cprm)  # range beyond the line
+        srv.remove_gather(gid["cmp"])
+        with pytest.raises(sb.SbError):
+            srv.submit_range(gid["cmp"], 0, 1, cprm)  # removed
+        assert srv.stats()["hbm_reserved"] < used0
+    print("served %d requests in %d launches" % (st["done"], st["launches"]))
+    for (kind, cb, ce, _), (out, stats) in outs:
+        assert out is not None, stats
+        if kind == "carried":
+            want = alone(sb, gpu_ctx, carried[cb])
+        else:
+            want = {k: v[cb:ce] for k, v in ref[kind].items()}
+        for k in ("index", "a", "b", "c", "coher", "stack"):
+            assert np.array_equal(out[k], want[k]), (kind, cb, ce, k)
