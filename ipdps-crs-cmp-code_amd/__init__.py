@@ -54,7 +54,8 @@ class HostResult(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("search_ms", C.c_double), ("finalize_ms", C.c_double), ("total_ms", C.c_double),
-                ("tuples", C.c_double), ("grid_blocks", C.c_int), ("block_threads", C.c_int)]
+                ("tuples", C.c_double), ("grid_blocks", C.c_int), ("block_threads", C.c_int),
+                ("variant", C.c_int), ("lds_staged", C.c_int)]
 
 
 class SynthSpec(C.Structure):

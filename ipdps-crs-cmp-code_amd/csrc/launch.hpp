@@ -8,9 +8,17 @@
 namespace sb {
 
 struct FastVariant {
-  int R;   // t0 rows per lane (block covers 64*R t0)
-  int CC;  // parameters evaluated together per trace pass
+  int R;       // t0 rows per lane (block covers 64*R t0)
+  int CC;      // parameters evaluated together per trace pass
+  int lds_kb;  // 0: windows from global memory (k_search_fast); else KiB of
+               // LDS slab per block (k_search_lds, CMP only)
+  int waves;   // waves per block (a block's waves split its parameters)
 };
+
+// LDS-staged CMP search: at most this many dynamic LDS bytes per block and
+// traces per CDP (the per-trace slab table lives in the same LDS).
+constexpr int kLdsMaxBytes = 64 * 1024;
+constexpr int kLdsMaxTraces = 2048;
 extern const FastVariant kFastVariants[];
 extern const int kNumFastVariants;
 
@@ -28,6 +36,7 @@ hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int s
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
 
 hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, hipStream_t st);
+hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& v, int nblocks, hipStream_t st);
 hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipStream_t st);
 hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);
 

@@ -296,6 +296,11 @@ extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, i
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     ctx_free(ctx, dg);
     if (e != hipSuccess) return hip_fail(e, "geometry");
+    // Largest scaled h: bounds the LDS slabs of the CMP search.
+    std::vector<SlotF> hs(ntr);
+    HIPR(hipMemcpy(hs.data(), G->d_cmp_slots, (size_t)ntr * sizeof(SlotF), hipMemcpyDeviceToHost));
+    G->cmp_hmax = 0.f;
+    for (int i = 0; i < ntr; i++) G->cmp_hmax = std::max(G->cmp_hmax, hs[i].h);
   }
 
   // CRS supergathers.
@@ -370,7 +375,7 @@ extern "C" int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out) 
   const size_t n = (size_t)ncdp * ns;
   int rc;
   void* p = nullptr;
-  if ((rc = ctx_alloc(ctx, n * (6 * 4 + 8), &p))) return rc;
+  if ((rc = ctx_alloc(ctx, n * (6 * 4 + 8) + 8, &p))) return rc;
   r->base = p;
   r->index = (int32_t*)p;
   r->a = (float*)((char*)p + n * 4);
@@ -379,6 +384,12 @@ extern "C" int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out) 
   r->coher = (float*)((char*)p + n * 16);
   r->stack = (float*)((char*)p + n * 20);
   r->keys = (unsigned long long*)((char*)p + n * 24);
+  r->status = (unsigned int*)((char*)p + n * 32);
+  hipError_t e = launch_fill_u64((unsigned long long*)r->status, 1, 0ull, ctx->stream);
+  if (e != hipSuccess) {
+    ctx_free(ctx, p);
+    return hip_fail(e, "result status");
+  }
   *out = r.release();
   return SB_OK;
 }
@@ -402,7 +413,10 @@ extern "C" int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_resul
            {out->c, r->c},         {out->coher, r->coher}, {out->stack, r->stack}};
   for (auto& x : m)
     if (x.h) HIPR(hipMemcpyAsync(x.h, x.d, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  unsigned int status = 0;
+  HIPR(hipMemcpyAsync(&status, r->status, 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPR(hipStreamSynchronize(ctx->stream));
+  if (status) return fail(SB_ERR_STATE, "a search block's LDS slab overflowed; results are incomplete");
   return SB_OK;
 }
 
@@ -412,28 +426,23 @@ namespace sb {
 // Work table: CDPs in descending cost, dealt round-robin over the 8 XCDs
 // (consecutive workgroups land on consecutive XCDs), so every block of a
 // CDP shares one L2 and heavy CDPs start first on every XCD.
-int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
-               int* nblocks, const std::string& key) {
-  if (!key.empty() && key == ctx->work_key) {
-    *nblocks = ctx->work_n;
-    return SB_OK;
-  }
-  ctx->work_key.clear();
-  const int ncdp = ce - cb;
+// Deals per-CDP item lists to the 8 XCDs (heaviest CDP first, each to the
+// least-loaded XCD), interleaves them so consecutive workgroups land on
+// consecutive XCDs, and uploads the table.
+static int deal_work(sb_ctx* ctx, std::vector<std::vector<WorkItem>>& per_cdp, const std::vector<int64_t>& cost,
+                     int* nblocks, const std::string& key) {
+  const int ncdp = (int)per_cdp.size();
   std::vector<int> order(ncdp);
-  std::iota(order.begin(), order.end(), cb);
+  std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
   constexpr int kXcd = 8;
   std::vector<std::vector<WorkItem>> lists(kXcd);
   std::vector<int64_t> load(kXcd, 0);
   for (int q = 0; q < ncdp; q++) {
-    const int cdp = order[q];
+    const int i = order[q];
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[x] += cost[cdp];
-    for (int t = 0; t < ntiles; t++)
-      for (int c = 0; c < nchunks; c++)
-        lists[x].push_back(WorkItem{cdp, t, c * pchunk, std::min(npar, (c + 1) * pchunk)});
+    load[x] += cost[i];
+    lists[x].insert(lists[x].end(), per_cdp[i].begin(), per_cdp[i].end());
   }
   size_t longest = 0;
   for (auto& l : lists) longest = std::max(longest, l.size());
@@ -460,6 +469,25 @@ int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, in
   ctx->work_key = key;
   ctx->work_n = *nblocks;
   return SB_OK;
+}
+
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
+               int* nblocks, const std::string& key) {
+  if (!key.empty() && key == ctx->work_key) {
+    *nblocks = ctx->work_n;
+    return SB_OK;
+  }
+  ctx->work_key.clear();
+  const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
+  std::vector<std::vector<WorkItem>> per_cdp(ce - cb);
+  std::vector<int64_t> c(ce - cb);
+  for (int i = cb; i < ce; i++) {
+    c[i - cb] = cost[i];
+    for (int t = 0; t < ntiles; t++)
+      for (int k = 0; k < nchunks; k++)
+        per_cdp[i - cb].push_back(WorkItem{i, t, k * pchunk, std::min(npar, (k + 1) * pchunk)});
+  }
+  return deal_work(ctx, per_cdp, c, nblocks, key);
 }
 
 }  // namespace sb
@@ -505,15 +533,33 @@ int check_range(const sb_ctx* ctx, const sb_gather* g, const sb_result* r, int c
   return SB_OK;
 }
 
-int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int cc) {
-  // Enough blocks to fill every CU several times; chunks a multiple of 4*CC.
+int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
+  // Enough blocks to fill every CU several times; chunks a multiple of
+  // `unit` (waves per block x parameters per wave pass).
   const int64_t want = (int64_t)ctx->num_cus * 8;
   const int64_t have = (int64_t)ncdp * ntiles;
-  int nchunks = (int)std::min<int64_t>((want + have - 1) / have, (npar + 4 * cc - 1) / (4 * cc));
+  int nchunks = (int)std::min<int64_t>((want + have - 1) / have, (npar + unit - 1) / unit);
   nchunks = std::max(1, nchunks);
   int pchunk = (npar + nchunks - 1) / nchunks;
-  pchunk = (pchunk + 4 * cc - 1) / (4 * cc) * (4 * cc);
+  pchunk = (pchunk + unit - 1) / unit * unit;
   return pchunk;
+}
+
+// Whether k_search_lds can stage every CDP of [cb, ce): the slab table of
+// the largest CDP must fit, and the widest slab of one trace in one round --
+// at most 64R t0 + sqrt(dc * h_max) samples of moveout (sqrt(a + x) - sqrt(a)
+// <= sqrt(x)) plus rounding margins and the window -- must fit half of what
+// is left, which the kernel requires.
+bool lds_fits(const sb_gather* g, const SearchArgs& a, const FastVariant& v, int cb, int ce) {
+  int smax = 0;
+  for (int i = cb; i < ce; i++) smax = std::max(smax, g->cdp_off[i + 1] - g->cdp_off[i]);
+  if (smax > kLdsMaxTraces) return false;
+  const int pb = g->store == SB_STORE_F32 ? 8 : 4;
+  const int hdr = ((4 * smax + 3) * 4 + 15) & ~15;
+  const int64_t capP = ((int64_t)v.lds_kb * 1024 - hdr) / pb - a.w;
+  const double dc = std::fabs((double)a.finc_c) * (v.waves * v.CC - 1) * 1.001 + 1e-6;
+  const double lmax = 64.0 * v.R + std::ceil(std::sqrt(dc * std::max(0.0, (double)g->cmp_hmax))) + 8 + a.w;
+  return capP > 0 && 2 * lmax <= (double)capP;
 }
 
 int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int mode, double params_per_tuple,
@@ -527,15 +573,32 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   double tuples = 0;
   for (int i = cb; i < ce; i++) tuples += (double)S[i] * g->ns * params_per_tuple;
   int variant = -1;
+  bool lds = false;
+  FastVariant v{};
+  a.status = r->status;
   if (mode == SB_MODE_FAST) {
     if (!fast_supported(a.w)) return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
     variant = ctx->fast_variant;
-    const FastVariant v = kFastVariants[variant];
-    const int tile = 64 * v.R;
-    const int ntiles = (g->ns + tile - 1) / tile;
-    const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.CC);
-    const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
-    if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+    v = kFastVariants[variant];
+    if (v.lds_kb > 0 && a.method == SB_METHOD_CMP && lds_fits(g, a, v, cb, ce)) {
+      lds = true;
+      a.lds_bytes = v.lds_kb * 1024;
+    }
+    {
+      // The global-memory kernel runs the first global shape when the LDS
+      // slabs do not apply.  Both kernels read the same work table.
+      if (v.lds_kb > 0 && !lds) {
+        variant = 0;
+        v = kFastVariants[0];
+      }
+      const int tile = 64 * v.R;
+      const int ntiles = (g->ns + tile - 1) / tile;
+      const int pchunk =
+          ctx->pchunk_override > 0 ? ctx->pchunk_override
+                                   : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
+      const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
+      if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+    }
   } else {
     const int ntiles = (g->ns + 63) / 64;
     const std::string key = work_key(g, a.method, 1, ntiles, 1, 1, cb, ce);
@@ -549,7 +612,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   if (mode == SB_MODE_FAST) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fill");
   HIPR(hipEventRecord(ctx->ev[1], ctx->stream));
-  if (mode == SB_MODE_FAST)
+  if (mode == SB_MODE_FAST && lds)
+    e = launch_search_lds(a, store, v, nblocks, ctx->stream);
+  else if (mode == SB_MODE_FAST)
     e = launch_search_fast(a, store, variant, nblocks, ctx->stream);
   else
     e = launch_search_exact(a, store, nblocks, ctx->stream);
@@ -564,6 +629,8 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
   ctx->last.block_threads = mode == SB_MODE_FAST ? 256 : 64;
+  ctx->last.variant = variant;
+  ctx->last.lds_staged = lds ? 1 : 0;
   ctx->timing_pending = true;
   return SB_OK;
 }

@@ -53,6 +53,7 @@ struct sb_gather {
   sb::SlotF* d_crs_slots_p = nullptr;  // unscaled FP32 geometry (CRS-DE exact mode)
   int64_t ncrs_slots = 0;
   std::vector<int32_t> cdp_off, crs_slot_off;  // host copies
+  float cmp_hmax = 0.f;                        // largest CMP slot h
   size_t bytes = 0;
 };
 
@@ -63,6 +64,7 @@ struct sb_result {
   int32_t* index = nullptr;
   float *a = nullptr, *b = nullptr, *c = nullptr, *coher = nullptr, *stack = nullptr;
   unsigned long long* keys = nullptr;
+  unsigned int* status = nullptr;  // set by a search that could not complete (checked on download)
 };
 
 namespace sb {

@@ -65,6 +65,8 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
     const int cands[] = {0, npar, npar / 2, npar / 4, npar / 8, npar / 16};
     for (int pc : cands) {
       if (pc != 0 && pc < 4 * cc) continue;
+      // LDS-staged variants (CMP only) size their own parameter chunks.
+      if (kFastVariants[v].lds_kb > 0 && (method != SB_METHOD_CMP || pc != 0)) continue;
       ctx->fast_variant = v;
       ctx->pchunk_override = pc;
       double tsum = 0;

@@ -1,0 +1,102 @@
+"""LDS-staged CMP search (k_search_lds) against the global-memory kernel.
+
+Both kernels do the same arithmetic on the same stored bits in the same trace
+order, so every output must be bit-identical -- for every store, window
+length, launch shape, parameter-grid direction and on gathers whose slabs
+need several LDS chunks per parameter round (wide offset range, long
+traces)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("index", "c", "coher", "stack")
+
+
+def _search(sb, ctx, g, prm, variant):
+    ctx.tune_set(variant, 0)
+    out = sb.search(ctx, g, prm)
+    return out, ctx.timing()
+
+
+def _lds_variants(sb, ctx):
+    n = ctx.tune_get()[2]
+    return [v for v in range(n) if v >= 4]
+
+
+@pytest.mark.parametrize("store", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("tau,dt_us", [(0.002, 2000), (0.0, 2000), (0.004, 2000), (0.006, 2000)])
+def test_lds_equals_global(sb, gpu_ctx, store, tau, dt_us):
+    print("case: lds vs global store=%s tau=%g" % (store, tau), flush=True)
+    st = {"f32": sb.STORE_F32, "f16": sb.STORE_F16, "bf16": sb.STORE_BF16}[store]
+    off, geom, s = sb.synth(0, 7, (20, 150), 733, dt_us, 17, h_max=1500.0, h_jitter=5.0, nevents=6)
+    g = sb.Gather(gpu_ctx, off, geom, s, dt_us, store=st)
+    try:
+        for c0, c1 in ((1.98e-7, 1.77e-6), (1.77e-6, 1.98e-7)):
+            prm = sb.cmp_params(c0, c1, 57, tau)
+            ref, t0 = _search(sb, gpu_ctx, g, prm, 0)
+            assert t0["lds_staged"] == 0
+            for v in _lds_variants(sb, gpu_ctx):
+                out, t = _search(sb, gpu_ctx, g, prm, v)
+                assert t["lds_staged"] == 1, (v, t)
+                for k in KEYS:
+                    assert np.array_equal(out[k], ref[k]), (v, c0, k)
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
+
+
+@pytest.mark.parametrize("store", ["f32", "f16"])
+def test_lds_many_chunks(sb, gpu_ctx, store):
+    """Half-offsets to 2.5 km, a wide c range and w = 7: every parameter
+    round's slabs span several LDS chunks; the results must not change."""
+    print("case: lds many chunks %s" % store, flush=True)
+    st = sb.STORE_F32 if store == "f32" else sb.STORE_F16
+    off, geom, s = sb.synth(0, 3, (90, 120), 2001, 2000, 5, h_max=2500.0, nevents=5)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=st)
+    try:
+        prm = sb.cmp_params(1.0e-7, 1.5e-6, 200, 0.006)
+        ref, _ = _search(sb, gpu_ctx, g, prm, 0)
+        for v in _lds_variants(sb, gpu_ctx):
+            out, t = _search(sb, gpu_ctx, g, prm, v)
+            assert t["lds_staged"] == 1, (v, t)
+            for k in KEYS:
+                assert np.array_equal(out[k], ref[k]), (v, k)
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
+
+
+def test_lds_falls_back_when_it_cannot_stage(sb, gpu_ctx):
+    """A CDP with more traces than the slab table holds runs on the global
+    kernel (same results), and CRS never uses the LDS variants."""
+    print("case: lds fallback", flush=True)
+    off, geom, s = sb.synth(0, 2, (2100, 2100), 301, 2000, 9, h_max=1500.0, nevents=3)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000)
+    try:
+        prm = sb.cmp_params(1.98e-7, 1.77e-6, 16, 0.002)
+        ref, _ = _search(sb, gpu_ctx, g, prm, 0)
+        out, t = _search(sb, gpu_ctx, g, prm, 4)
+        assert t["lds_staged"] == 0 and t["variant"] == 0, t
+        for k in KEYS:
+            assert np.array_equal(out[k], ref[k]), k
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
+
+
+def test_lds_range_search(sb, gpu_ctx):
+    print("case: lds range search", flush=True)
+    off, geom, s = sb.synth(0, 9, (30, 90), 801, 2000, 23, h_max=1500.0, nevents=5)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=sb.STORE_F16)
+    try:
+        prm = sb.cmp_params(1.98e-7, 1.77e-6, 64, 0.002)
+        gpu_ctx.tune_set(5, 0)
+        full = sb.search(gpu_ctx, g, prm)
+        part = sb.search(gpu_ctx, g, prm, cdp_range=(2, 6))
+        assert gpu_ctx.timing()["lds_staged"] == 1
+        for k in KEYS:
+            assert np.array_equal(part[k], full[k][2:6]), k
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
