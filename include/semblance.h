@@ -242,6 +242,31 @@ int sb_tune_set(sb_ctx* ctx, int variant, int pchunk);
 int sb_tune_get(const sb_ctx* ctx, int* variant, int* pchunk, int* nvariants);
 int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int reps);
 
+/* Context options.  SB_OPT_PERSISTENT = 1: the global-memory fast search
+ * runs as a persistent grid (as many workgroups as the CUs hold at once,
+ * each XCD's workgroups pulling that XCD's work items from their own
+ * counter) instead of one workgroup per work item.  Results are the same. */
+enum { SB_OPT_PERSISTENT = 1 };
+int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
+int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);
+
+/* --------------------------------------------------------------- graphs */
+/* One search captured into a HIP graph and replayed without host work:
+ * sb_graph_create runs the search once (building the work table), then
+ * captures every kernel of it -- key clear, search, winner re-evaluation --
+ * on the ctx stream; sb_graph_launch replays it there, reading the gather
+ * and writing r as an eager *_run_range call would (download with
+ * sb_result_download).  kind: SB_METHOD_CMP (sb_cmp_params), SB_METHOD_CRS
+ * (sb_crs_params) or SB_GRAPH_CRS_DE (sb_de_params).  The graph keeps its
+ * own copy of the work table; ctx, gather and result must outlive it.  No
+ * reference counterpart (the reference relaunches per CDP). */
+enum { SB_GRAPH_CRS_DE = 2 };
+typedef struct sb_graph sb_graph;
+int sb_graph_create(sb_ctx* ctx, const sb_gather* g, int kind, const void* params, int cdp_begin, int cdp_end,
+                    sb_result* r, sb_graph** out);
+int sb_graph_launch(sb_graph* gr);
+void sb_graph_destroy(sb_graph* gr);
+
 /* ------------------------------------------------------ synthetic data */
 /* Seeded synthetic gathers for the benchmark configurations (host, OpenMP).
  * kind 0: 2D line, CMP events t^2 = t0^2 + C h^2 (h = half-offset);

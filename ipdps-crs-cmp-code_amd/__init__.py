@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libsemblance.so")
 METHOD_CMP, METHOD_CRS = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 MODE_FAST, MODE_EXACT = 0, 1
+OPT_PERSISTENT = 1
+GRAPH_CRS_DE = 2
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
 
 GEOM_DTYPE = np.dtype([("cdp", "<i4"), ("scalco", "<i4"), ("sx", "<i4"), ("sy", "<i4"), ("gx", "<i4"),
@@ -91,6 +93,7 @@ EXPORTS = [
     "sb_crs_de_run_range", "sb_cmp", "sb_crs", "sb_crs_de",
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
+    "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
 ]
 
 
@@ -160,6 +163,12 @@ def lib():
         L.sb_server_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_size_t)]
         L.sb_server_destroy.argtypes = [P]
         L.sb_server_destroy.restype = None
+        L.sb_ctx_set_option.argtypes = [P, C.c_int, C.c_int]
+        L.sb_ctx_get_option.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.sb_graph_create.argtypes = [P, P, C.c_int, P, C.c_int, C.c_int, P, C.POINTER(P)]
+        L.sb_graph_launch.argtypes = [P]
+        L.sb_graph_destroy.argtypes = [P]
+        L.sb_graph_destroy.restype = None
         _lib = L
     return _lib
 
@@ -272,6 +281,14 @@ class Context:
         check(lib().sb_last_timing(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
 
+    def set_option(self, option, value):
+        check(lib().sb_ctx_set_option(self.h, option, int(value)))
+
+    def get_option(self, option):
+        v = C.c_int()
+        check(lib().sb_ctx_get_option(self.h, option, C.byref(v)))
+        return v.value
+
     def tune_set(self, variant, pchunk=0):
         check(lib().sb_tune_set(self.h, variant, pchunk))
 
@@ -372,6 +389,34 @@ def search(ctx, gather, params, cdp_range=None):
         return r.download()
     finally:
         r.close()
+
+
+class Graph:
+    """sb_graph: one search captured into a HIP graph; launch() replays it
+    into `result` (download with result.download())."""
+
+    def __init__(self, ctx, gather, params, result, cdp_range=None):
+        kind = METHOD_CMP if isinstance(params, CmpParams) else METHOD_CRS if isinstance(params, CrsParams) \
+            else GRAPH_CRS_DE
+        cb, ce = (0, gather.ncdp) if cdp_range is None else cdp_range
+        self.ctx, self._keep = ctx, (gather, params, result)
+        self.h = C.c_void_p()
+        check(lib().sb_graph_create(ctx.h, gather.h, kind, C.cast(C.pointer(params), C.c_void_p), int(cb), int(ce),
+                                    result.h, C.byref(self.h)))
+
+    def launch(self):
+        check(lib().sb_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            lib().sb_graph_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def tune(ctx, gather, params, reps=2):

@@ -13,11 +13,13 @@ namespace sb {
 // ------------------------------------------------------------------ stores
 // Samples of row r, element i, restored to FP32.
 struct LoadF32 {
+  static constexpr bool kQuantized = false;
   const float* p;
   int64_t stride;
   __device__ float operator()(int64_t row, int i) const { return p[row * stride + i]; }
 };
 struct LoadF16 {
+  static constexpr bool kQuantized = true;
   const __half* p;
   const float* scale;
   int64_t stride;
@@ -26,6 +28,7 @@ struct LoadF16 {
   }
 };
 struct LoadBF16 {
+  static constexpr bool kQuantized = true;
   const uint16_t* p;
   const float* scale;
   int64_t stride;

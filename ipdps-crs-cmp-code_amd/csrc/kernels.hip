@@ -16,6 +16,8 @@
 // every (CDP, t0) is re-evaluated by k_finalize in the reference's precision
 // and operation order.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
@@ -186,7 +188,11 @@ __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
 // ---------------------------------------------- reference-order evaluation
 // One (CDP, t0, c) of CMP exactly as CMP/OpenMP/src/main.cpp:150-203: float,
 // no contraction, correctly rounded sqrt/div, traces in gather order.
-template <class LD>
+// KAHAN: the window, energy and stack sums carry a compensation term
+// (Kahan summation) -- used to report 16-bit-store results, whose inputs
+// already differ from the reference's, so that FP32 accumulation adds no
+// error of its own; FP32 stores keep the reference's plain sums (and bits).
+template <bool KAHAN = false, class LD>
 __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int cid, float& sem,
                                float& stk) {
 #pragma clang fp contract(off)
@@ -199,6 +205,19 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
 #pragma unroll
   for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.f;
   float den = 0.f, lin = 0.f, m = 0.f;
+  float cnum[SB_MAX_W], cden = 0.f, clin = 0.f;  // Kahan compensations
+#pragma unroll
+  for (int j = 0; j < SB_MAX_W; j++) cnum[j] = 0.f;
+  auto add = [](float& sum, float& comp, float v) {
+    if constexpr (KAHAN) {
+      const float y = v - comp;
+      const float t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
+    } else {
+      sum += v;
+    }
+  };
   int err = 0;
   const int r0 = a.cdp_off[cdp], r1 = a.cdp_off[cdp + 1];
   for (int row = r0; row < r1; row++) {
@@ -216,10 +235,10 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
           s1 = ld(row, e + j + 1);
           float d = s1 - s0;
           float v = d * x + s0;
-          num[j] += v;
+          add(num[j], cnum[j], v);
           float vv = v * v;
-          den += vv;
-          lin += v;
+          add(den, cden, vv);
+          add(lin, clin, v);
         }
       }
       m += 1.0f;
@@ -393,7 +412,7 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
   if (a.method == SB_METHOD_CMP) {
     float sem = 0.f, stk = 0.f;
-    if (p >= 0) eval_cmp_exact(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_cmp_exact<LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.f)
       write_cmp(a, o, p, sem, stk);
     else
@@ -412,12 +431,12 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
 }
 
 // ---------------------------------------------------- SB_MODE_FAST search
+// One work item (CDP, t0 tile, parameter chunk) by one block of 4 waves.
 template <int W, int R, int CC, bool CRS, int STORE>
-__global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
+__device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem wi,
+                                            unsigned long long (&s_best)[4][64 * R]) {
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
-  __shared__ unsigned long long s_best[4][64 * R];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const WorkItem wi = a.work[blockIdx.x];
   if (wi.cdp < 0) return;
   const int cdp = wi.cdp;
   const int k0 = a.slot_off[cdp];
@@ -619,6 +638,30 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
   }
 }
 
+// One block per work item, or (a.queue set) a persistent grid: the blocks
+// of XCD x = blockIdx % 8 (workgroups are dealt to the 8 XCDs round-robin)
+// take the work table's items x, x+8, x+16, ... -- the CDPs dealt to that
+// XCD -- in order from the XCD's own counter until the table is exhausted.
+template <int W, int R, int CC, bool CRS, int STORE>
+__global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
+  __shared__ unsigned long long s_best[4][64 * R];
+  if (a.queue == nullptr) {
+    search_item<W, R, CC, CRS, STORE>(a, a.work[blockIdx.x], s_best);
+    return;
+  }
+  __shared__ unsigned int s_next;
+  const unsigned x = blockIdx.x & 7u;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(&a.queue[x], 1u);
+    __syncthreads();
+    const int64_t idx = (int64_t)s_next * 8 + x;
+    __syncthreads();  // s_next read by every thread before it is rewritten
+    if (idx >= a.nwork) break;
+    search_item<W, R, CC, CRS, STORE>(a, a.work[idx], s_best);
+    __syncthreads();  // wave 0 has read s_best before the next item writes it
+  }
+}
+
 // ------------------------------------------------------------- launchers
 #define SB_CHECK_LAUNCH() (hipGetLastError())
 
@@ -691,24 +734,34 @@ hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipS
   return exact_with(a, LoadBF16{(const uint16_t*)a.samples, a.scale, a.stride}, nblocks, st);
 }
 
-template <int W, int R, int CC, bool CRS>
-static hipError_t fast_store(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
-  if (store == SB_STORE_F32)
-    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_F32>), dim3(nblocks), dim3(256), 0, st, a);
-  else if (store == SB_STORE_F16)
-    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_F16>), dim3(nblocks), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, SB_STORE_BF16>), dim3(nblocks), dim3(256), 0, st, a);
+template <int W, int R, int CC, bool CRS, int STORE>
+static hipError_t fast_launch(const SearchArgs& a, int nblocks, int num_cus, hipStream_t st) {
+  auto kern = k_search_fast<W, R, CC, CRS, STORE>;
+  int grid = nblocks;
+  if (a.queue) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 1;
+    const int full = std::max(8, num_cus * per / 8 * 8);
+    grid = std::min((nblocks + 7) / 8 * 8, full);
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, a);
   return SB_CHECK_LAUNCH();
 }
 
+template <int W, int R, int CC, bool CRS>
+static hipError_t fast_store(const SearchArgs& a, int store, int nblocks, int num_cus, hipStream_t st) {
+  if (store == SB_STORE_F32) return fast_launch<W, R, CC, CRS, SB_STORE_F32>(a, nblocks, num_cus, st);
+  if (store == SB_STORE_F16) return fast_launch<W, R, CC, CRS, SB_STORE_F16>(a, nblocks, num_cus, st);
+  return fast_launch<W, R, CC, CRS, SB_STORE_BF16>(a, nblocks, num_cus, st);
+}
+
 template <int R, int CC, bool CRS>
-static hipError_t fast_w(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
+static hipError_t fast_w(const SearchArgs& a, int store, int nblocks, int num_cus, hipStream_t st) {
   switch (a.w) {
-    case 1: return fast_store<1, R, CC, CRS>(a, store, nblocks, st);
-    case 3: return fast_store<3, R, CC, CRS>(a, store, nblocks, st);
-    case 5: return fast_store<5, R, CC, CRS>(a, store, nblocks, st);
-    case 7: return fast_store<7, R, CC, CRS>(a, store, nblocks, st);
+    case 1: return fast_store<1, R, CC, CRS>(a, store, nblocks, num_cus, st);
+    case 3: return fast_store<3, R, CC, CRS>(a, store, nblocks, num_cus, st);
+    case 5: return fast_store<5, R, CC, CRS>(a, store, nblocks, num_cus, st);
+    case 7: return fast_store<7, R, CC, CRS>(a, store, nblocks, num_cus, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -719,14 +772,15 @@ const FastVariant kFastVariants[] = {{2, 4, 0, 4},  {1, 4, 0, 4},  {4, 2, 0, 4},
                                      {1, 4, 32, 8}, {2, 2, 48, 8}, {1, 2, 24, 8}, {1, 4, 20, 4}};
 const int kNumFastVariants = sizeof(kFastVariants) / sizeof(kFastVariants[0]);
 
-hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, hipStream_t st) {
+hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, int num_cus, hipStream_t st) {
   if (nblocks <= 0) return hipSuccess;
   const bool crs = a.method == SB_METHOD_CRS;
+  const int n = nblocks, c = num_cus;
   switch (variant) {
-    case 0: return crs ? fast_w<2, 4, true>(a, store, nblocks, st) : fast_w<2, 4, false>(a, store, nblocks, st);
-    case 1: return crs ? fast_w<1, 4, true>(a, store, nblocks, st) : fast_w<1, 4, false>(a, store, nblocks, st);
-    case 2: return crs ? fast_w<4, 2, true>(a, store, nblocks, st) : fast_w<4, 2, false>(a, store, nblocks, st);
-    case 3: return crs ? fast_w<2, 2, true>(a, store, nblocks, st) : fast_w<2, 2, false>(a, store, nblocks, st);
+    case 0: return crs ? fast_w<2, 4, true>(a, store, n, c, st) : fast_w<2, 4, false>(a, store, n, c, st);
+    case 1: return crs ? fast_w<1, 4, true>(a, store, n, c, st) : fast_w<1, 4, false>(a, store, n, c, st);
+    case 2: return crs ? fast_w<4, 2, true>(a, store, n, c, st) : fast_w<4, 2, false>(a, store, n, c, st);
+    case 3: return crs ? fast_w<2, 2, true>(a, store, n, c, st) : fast_w<2, 2, false>(a, store, n, c, st);
     default: return hipErrorInvalidValue;
   }
 }

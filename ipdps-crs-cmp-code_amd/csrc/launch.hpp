@@ -35,7 +35,9 @@ hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int s
                         hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
 
-hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, hipStream_t st);
+// a.queue set: persistent grid (at most what the CUs hold at once, a
+// multiple of 8) pulling the nblocks work items; else one block per item.
+hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, int num_cus, hipStream_t st);
 hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& v, int nblocks, hipStream_t st);
 hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipStream_t st);
 hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);

@@ -533,6 +533,14 @@ int check_range(const sb_ctx* ctx, const sb_gather* g, const sb_result* r, int c
   return SB_OK;
 }
 
+// Timing events are not recorded while a graph is being captured.
+hipError_t record(sb_ctx* ctx, int i) { return ctx->capturing ? hipSuccess : hipEventRecord(ctx->ev[i], ctx->stream); }
+
+// The work table a launch reads: a graph being captured gets its own copy.
+const WorkItem* work_table(const sb_ctx* ctx) {
+  return (const WorkItem*)(ctx->graph_work ? ctx->graph_work : ctx->work_buf);
+}
+
 int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
   // Enough blocks to fill every CU several times; chunks a multiple of
   // `unit` (waves per block x parameters per wave pass).
@@ -604,27 +612,35 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
     const std::string key = work_key(g, a.method, 1, ntiles, 1, 1, cb, ce);
     if ((rc = build_work(ctx, cost, cb, ce, ntiles, 1, 1, &nblocks, key))) return rc;
   }
-  a.work = (const WorkItem*)ctx->work_buf;
+  a.work = work_table(ctx);
   a.nwork = nblocks;
   const int64_t nout = (int64_t)(ce - cb) * g->ns;
-  HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPR(record(ctx, 0));
   hipError_t e = hipSuccess;
   if (mode == SB_MODE_FAST) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fill");
-  HIPR(hipEventRecord(ctx->ev[1], ctx->stream));
+  if (mode == SB_MODE_FAST && !lds && ctx->persistent) {
+    // Persistent grid: 8 per-XCD item counters, cleared by a kernel (not a
+    // memset, so that the sequence can be captured into a graph).
+    if (!ctx->queue_buf && (rc = ctx_alloc(ctx, 64, &ctx->queue_buf))) return rc;
+    a.queue = (unsigned int*)ctx->queue_buf;
+    e = launch_fill_u64((unsigned long long*)ctx->queue_buf, 4, 0ull, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "queue clear");
+  }
+  HIPR(record(ctx, 1));
   if (mode == SB_MODE_FAST && lds)
     e = launch_search_lds(a, store, v, nblocks, ctx->stream);
   else if (mode == SB_MODE_FAST)
-    e = launch_search_fast(a, store, variant, nblocks, ctx->stream);
+    e = launch_search_fast(a, store, variant, nblocks, ctx->num_cus, ctx->stream);
   else
     e = launch_search_exact(a, store, nblocks, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "search launch");
-  HIPR(hipEventRecord(ctx->ev[2], ctx->stream));
+  HIPR(record(ctx, 2));
   if (mode == SB_MODE_FAST) {
     e = launch_finalize(a, store, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "finalize launch");
   }
-  HIPR(hipEventRecord(ctx->ev[3], ctx->stream));
+  HIPR(record(ctx, 3));
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
@@ -632,6 +648,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   ctx->last.variant = variant;
   ctx->last.lds_staged = lds ? 1 : 0;
   ctx->timing_pending = true;
+  ctx->timing_graph = false;
   return SB_OK;
 }
 
@@ -805,19 +822,21 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
   int nblocks = 0;
   if ((rc = build_work(ctx, cost, cb, ce, ntiles, 1, 1, &nblocks, work_key(g, SB_METHOD_CRS, 2, ntiles, 1, 1, cb, ce))))
     return rc;
-  a.work = (const WorkItem*)ctx->work_buf;
+  a.work = work_table(ctx);
   a.nwork = nblocks;
-  HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
-  HIPR(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPR(record(ctx, 0));
+  HIPR(record(ctx, 1));
   hipError_t e = launch_de(a, d, g->store, p->mode == SB_MODE_EXACT, nblocks, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "CRS-DE launch");
-  HIPR(hipEventRecord(ctx->ev[2], ctx->stream));
-  HIPR(hipEventRecord(ctx->ev[3], ctx->stream));
+  HIPR(record(ctx, 2));
+  HIPR(record(ctx, 3));
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
   ctx->last.block_threads = 256;
+  ctx->last.variant = -1;
   ctx->timing_pending = true;
+  ctx->timing_graph = false;
   return SB_OK;
 }
 
@@ -828,9 +847,13 @@ extern "C" int sb_last_timing(const sb_ctx* cctx, sb_timing* t) {
     HIPR(hipSetDevice(ctx->device));
     HIPR(hipEventSynchronize(ctx->ev[3]));
     float a = 0, b = 0, c = 0;
-    HIPR(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]));
-    HIPR(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
     HIPR(hipEventElapsedTime(&c, ctx->ev[0], ctx->ev[3]));
+    if (ctx->timing_graph) {  // a graph replay: only its whole span is timed
+      a = c;
+    } else {
+      HIPR(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]));
+      HIPR(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
+    }
     ctx->last.search_ms = a;
     ctx->last.finalize_ms = b;
     ctx->last.total_ms = c;
@@ -864,3 +887,96 @@ extern "C" int sb_crs_de(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p,
   return run_and_download(ctx, g, p, out, sb_crs_de_run);
 }
 
+
+// ------------------------------------------------------------- options
+extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
+  if (!ctx) return fail(SB_ERR_INVALID, "null ctx");
+  if (option == SB_OPT_PERSISTENT) {
+    ctx->persistent = value ? 1 : 0;
+    return SB_OK;
+  }
+  return fail(SB_ERR_INVALID, "unknown option");
+}
+
+extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
+  if (!ctx || !value) return fail(SB_ERR_INVALID, "null argument");
+  if (option == SB_OPT_PERSISTENT) {
+    *value = ctx->persistent;
+    return SB_OK;
+  }
+  return fail(SB_ERR_INVALID, "unknown option");
+}
+
+// -------------------------------------------------------------- graphs
+struct sb_graph {
+  sb_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* work = nullptr;  // the graph's copy of the work table
+};
+
+extern "C" void sb_graph_destroy(sb_graph* gr) {
+  if (!gr) return;
+  (void)hipSetDevice(gr->ctx->device);
+  (void)hipStreamSynchronize(gr->ctx->stream);
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  ctx_free(gr->ctx, gr->work);
+  delete gr;
+}
+
+extern "C" int sb_graph_create(sb_ctx* ctx, const sb_gather* g, int kind, const void* params, int cb, int ce,
+                               sb_result* r, sb_graph** out) {
+  if (!ctx || !g || !params || !r || !out) return fail(SB_ERR_INVALID, "sb_graph_create: null argument");
+  *out = nullptr;
+  auto run = [&]() -> int {
+    if (kind == SB_METHOD_CMP) return sb_cmp_run_range(ctx, g, (const sb_cmp_params*)params, cb, ce, r);
+    if (kind == SB_METHOD_CRS) return sb_crs_run_range(ctx, g, (const sb_crs_params*)params, cb, ce, r);
+    if (kind == SB_GRAPH_CRS_DE) return sb_crs_de_run_range(ctx, g, (const sb_de_params*)params, cb, ce, r);
+    return fail(SB_ERR_INVALID, "sb_graph_create: bad kind");
+  };
+  // 1. eager run: validates the arguments and builds the work table.
+  int rc = run();
+  if (rc) return rc;
+  HIPR(hipSetDevice(ctx->device));
+  HIPR(hipStreamSynchronize(ctx->stream));
+  std::unique_ptr<sb_graph, void (*)(sb_graph*)> G(new sb_graph, sb_graph_destroy);
+  G->ctx = ctx;
+  const size_t bytes = std::max<size_t>((size_t)ctx->work_n, 1) * sizeof(WorkItem);
+  if ((rc = ctx_alloc(ctx, bytes, &G->work))) return rc;
+  HIPR(hipMemcpy(G->work, ctx->work_buf, bytes, hipMemcpyDeviceToDevice));
+  // 2. capture the same calls: the table is cached (same key), so nothing
+  // but kernel launches reaches the stream.
+  ctx->graph_work = G->work;
+  ctx->capturing = true;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = run();
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+    G->graph = graph;
+    if (e2 != hipSuccess && !rc) rc = hip_fail(e2, "graph capture");
+  } else {
+    rc = hip_fail(e, "graph capture");
+  }
+  ctx->capturing = false;
+  ctx->graph_work = nullptr;
+  ctx->timing_pending = false;
+  if (rc) return rc;
+  HIPR(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+  *out = G.release();
+  return SB_OK;
+}
+
+extern "C" int sb_graph_launch(sb_graph* gr) {
+  if (!gr) return fail(SB_ERR_INVALID, "null graph");
+  sb_ctx* ctx = gr->ctx;
+  HIPR(hipSetDevice(ctx->device));
+  std::lock_guard<std::mutex> lk(ctx->run_mu);
+  HIPR(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPR(hipGraphLaunch(gr->exec, ctx->stream));
+  HIPR(hipEventRecord(ctx->ev[3], ctx->stream));
+  ctx->timing_pending = true;
+  ctx->timing_graph = true;
+  return SB_OK;
+}

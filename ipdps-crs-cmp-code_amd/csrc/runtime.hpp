@@ -21,6 +21,7 @@ struct sb_ctx {
   hipEvent_t ev[4] = {};
   sb_timing last{};
   bool timing_pending = false;
+  bool timing_graph = false;  // the pending timing is a graph replay
   int num_cus = 256;
   // launch shape of the fast search (autotuned or set by sb_tune_set)
   int fast_variant = 0;
@@ -32,7 +33,10 @@ struct sb_ctx {
   std::string work_key;  // identifies the table in work_buf
   int work_n = 0;
   unsigned long long gather_serial = 0;
-  void* queue_buf = nullptr;
+  void* queue_buf = nullptr;   // persistent-grid item counters
+  int persistent = 0;          // SB_OPT_PERSISTENT
+  bool capturing = false;      // a graph is being captured on `stream`
+  void* graph_work = nullptr;  // work table copy owned by the graph being captured
 };
 
 struct sb_gather {
