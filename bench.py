@@ -302,6 +302,48 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4):
     return res
 
 
+def launch_modes(sb, ctx, g, prm, steps=3):
+    """Search ms of the same step under the other launch modes (same
+    results): the persistent per-XCD grid, the LDS-staged kernel (best of its
+    shapes) and a replayed HIP graph of the tuned shape (whole step)."""
+    v0, pc0, nv = ctx.tune_get()
+    r = sb.DeviceResult(ctx, g.ncdp, g.ns)
+    out = {}
+    try:
+        def time_it():
+            sb.run(ctx, g, prm, r)
+            ms = []
+            for _ in range(steps):
+                sb.run(ctx, g, prm, r)
+                ms.append(ctx.timing()["search_ms"])
+            return float(np.median(ms))
+        ctx.set_option(sb.OPT_PERSISTENT, 1)
+        out["persistent_search_ms"] = time_it()
+        ctx.set_option(sb.OPT_PERSISTENT, 0)
+        best = None
+        for v in range(4, nv):
+            ctx.tune_set(v, 0)
+            t = time_it()
+            if ctx.timing()["lds_staged"] and (best is None or t < best[1]):
+                best = (v, t)
+        if best:
+            out["lds_staged_search_ms"], out["lds_variant"] = best[1], best[0]
+        ctx.tune_set(v0, pc0)
+        with sb.Graph(ctx, g, prm, r) as gr:
+            gr.launch()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                gr.launch()
+            ctx.sync()
+            out["graph_ms_per_step"] = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        ctx.set_option(sb.OPT_PERSISTENT, 0)
+        ctx.tune_set(v0, pc0)
+        r.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +357,7 @@ def main():
     ap.add_argument("--no-latency", action="store_true", help="skip the 1-CDP panel latency probe")
     ap.add_argument("--no-side", action="store_true", help="skip the CRS grid and CRS-DE side measurements")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop load test")
+    ap.add_argument("--no-modes", action="store_true", help="skip the persistent / LDS / graph launch-mode timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -343,9 +386,12 @@ def main():
             fin_ms = float(np.mean([t["finalize_ms"] for t in tk]))
             results[store] = dict(value=tuples * args.steps / wall, ms_per_step=wall / args.steps * 1e3,
                                   search_ms=search_ms, finalize_ms=fin_ms, tuples=tuples, tune=ctx.tune_get()[:2],
-                                  blocks=tk[0]["grid_blocks"])
+                                  blocks=tk[0]["grid_blocks"], lds_staged=tk[0]["lds_staged"])
             print("bench: %s %.4g evals/s, search %.3f ms, finalize %.3f ms" %
                   (store, results[store]["value"], search_ms, fin_ms), flush=True)
+            if not args.no_modes:
+                results[store]["modes"] = launch_modes(sb, ctx, g, prm)
+                print("bench: %s launch modes %s" % (store, results[store]["modes"]), flush=True)
             g.close()
         p50, p99 = (None, None) if args.no_latency else panel_latency(sb, ctx, cdp_off, geom, samples)
     if "f32" not in results or "f16" not in results:
@@ -373,14 +419,19 @@ def main():
         "data": "synthetic (seeded Ricker events + noise, generated in-process)",
         "config": {"workload": "CMP 2000 CDPs fold 30-120 ns 1501 dt 2ms nc 256 tau 2ms (FP32 store)",
                    "ncdp": CFG["ncdp"], "ns": CFG["ns"], "nc": CFG["nc"], "tuples_per_step": f32["tuples"],
-                   "launch": {"variant": f32["tune"][0], "pchunk": f32["tune"][1], "blocks": f32["blocks"]}},
+                   "launch": {"variant": f32["tune"][0], "pchunk": f32["tune"][1], "blocks": f32["blocks"],
+                              "lds_staged": f32["lds_staged"]}},
         "fp16_store": {"value": results["f16"]["value"], "ms_per_step": results["f16"]["ms_per_step"],
-                       "speedup_vs_f32": results["f16"]["value"] / f32["value"]},
+                       "speedup_vs_f32": results["f16"]["value"] / f32["value"],
+                       "launch": {"variant": results["f16"]["tune"][0], "pchunk": results["f16"]["tune"][1]}},
+        "launch_modes_ms": {k: results[k].get("modes") for k in ("f32", "f16")},
         "panel_latency_ms": {"p50": p50, "p99": p99, "what": "1-CDP CMP panel, upload+search+download, sequential"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VECTOR_TFLOPS, "traffic": traffic,
                      "kernel": "k_search_fast (CMP, W=3)", "kernel_ms": f32["search_ms"],
-                     "flop_per_eval": FLOP_PER_TUPLE_CMP},
+                     "flop_per_eval": FLOP_PER_TUPLE_CMP,
+                     "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s; "
+                             "the kernel is bound by the vector-memory address unit (TA ~96% busy)"},
     }
     if not args.no_cpu:
         print("bench: CPU baseline", flush=True)

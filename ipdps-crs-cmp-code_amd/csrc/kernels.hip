@@ -638,17 +638,21 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
   }
 }
 
-// One block per work item, or (a.queue set) a persistent grid: the blocks
-// of XCD x = blockIdx % 8 (workgroups are dealt to the 8 XCDs round-robin)
-// take the work table's items x, x+8, x+16, ... -- the CDPs dealt to that
-// XCD -- in order from the XCD's own counter until the table is exhausted.
+// One block per work item.
 template <int W, int R, int CC, bool CRS, int STORE>
 __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
   __shared__ unsigned long long s_best[4][64 * R];
-  if (a.queue == nullptr) {
-    search_item<W, R, CC, CRS, STORE>(a, a.work[blockIdx.x], s_best);
-    return;
-  }
+  search_item<W, R, CC, CRS, STORE>(a, a.work[blockIdx.x], s_best);
+}
+
+// Persistent grid (a separate kernel, so the per-item kernel's register
+// allocation is not shaped by this loop): the blocks of XCD
+// x = blockIdx % 8 (workgroups are dealt to the 8 XCDs round-robin) take the
+// work table's items x, x+8, x+16, ... -- the CDPs dealt to that XCD -- in
+// order from the XCD's own counter until the table is exhausted.
+template <int W, int R, int CC, bool CRS, int STORE>
+__global__ __launch_bounds__(256) void k_search_fast_persistent(SearchArgs a) {
+  __shared__ unsigned long long s_best[4][64 * R];
   __shared__ unsigned int s_next;
   const unsigned x = blockIdx.x & 7u;
   for (;;) {
@@ -736,14 +740,15 @@ hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipS
 
 template <int W, int R, int CC, bool CRS, int STORE>
 static hipError_t fast_launch(const SearchArgs& a, int nblocks, int num_cus, hipStream_t st) {
-  auto kern = k_search_fast<W, R, CC, CRS, STORE>;
-  int grid = nblocks;
-  if (a.queue) {
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 1;
-    const int full = std::max(8, num_cus * per / 8 * 8);
-    grid = std::min((nblocks + 7) / 8 * 8, full);
+  if (!a.queue) {
+    hipLaunchKernelGGL((k_search_fast<W, R, CC, CRS, STORE>), dim3(nblocks), dim3(256), 0, st, a);
+    return SB_CHECK_LAUNCH();
   }
+  auto kern = k_search_fast_persistent<W, R, CC, CRS, STORE>;
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 1;
+  const int full = std::max(8, num_cus * per / 8 * 8);
+  const int grid = std::min((nblocks + 7) / 8 * 8, full);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, a);
   return SB_CHECK_LAUNCH();
 }

@@ -167,6 +167,8 @@ extern "C" void sb_ctx_destroy(sb_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx_release_all(ctx);
+  if (ctx->work_host) (void)hipHostFree(ctx->work_host);
+  if (ctx->work_host_ev) (void)hipEventDestroy(ctx->work_host_ev);
   for (auto& ev : ctx->ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -460,11 +462,24 @@ static int deal_work(sb_ctx* ctx, std::vector<std::vector<WorkItem>>& per_cdp, c
     if (rc) return rc;
     ctx->work_cap = bytes * 2;
   }
-  if (!items.empty())
-    HIPR(hipMemcpyAsync(ctx->work_buf, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
-                        ctx->stream));
-  // The copy reads a host vector that dies here: wait for it.
-  HIPR(hipStreamSynchronize(ctx->stream));
+  if (!items.empty()) {
+    // Staged through a pinned host buffer: a true asynchronous copy, ordered
+    // on the stream.  The buffer is rewritten only once the previous table's
+    // copy has completed (work_ev).
+    const size_t n = items.size() * sizeof(WorkItem);
+    if (ctx->work_host_ev) HIPR(hipEventSynchronize(ctx->work_host_ev));
+    if (n > ctx->work_host_cap) {
+      if (ctx->work_host) (void)hipHostFree(ctx->work_host);
+      ctx->work_host = nullptr;
+      ctx->work_host_cap = 0;
+      HIPR(hipHostMalloc(&ctx->work_host, n * 2, hipHostMallocDefault));
+      ctx->work_host_cap = n * 2;
+    }
+    std::memcpy(ctx->work_host, items.data(), n);
+    HIPR(hipMemcpyAsync(ctx->work_buf, ctx->work_host, n, hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->work_host_ev) HIPR(hipEventCreateWithFlags(&ctx->work_host_ev, hipEventDisableTiming));
+    HIPR(hipEventRecord(ctx->work_host_ev, ctx->stream));
+  }
   *nblocks = (int)items.size();
   ctx->work_key = key;
   ctx->work_n = *nblocks;

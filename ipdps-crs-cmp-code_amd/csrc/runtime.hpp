@@ -31,6 +31,9 @@ struct sb_ctx {
   void* work_buf = nullptr;
   size_t work_cap = 0;
   std::string work_key;  // identifies the table in work_buf
+  void* work_host = nullptr;  // pinned staging of the table upload
+  size_t work_host_cap = 0;
+  hipEvent_t work_host_ev = nullptr;  // the last upload from work_host has completed
   int work_n = 0;
   unsigned long long gather_serial = 0;
   void* queue_buf = nullptr;   // persistent-grid item counters

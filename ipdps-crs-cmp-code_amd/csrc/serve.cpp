@@ -40,9 +40,10 @@ using namespace sb;
 namespace {
 
 constexpr int kSlots = 4;  // matches the 4 hardware queues a process gets by default
-// Slot 0 only takes small requests (at most kSmallEvals evaluations, a few ms
-// of GPU time), so interactive panels never wait behind a long search.
-constexpr double kSmallEvals = 2e10;
+// Slot 0 only takes small requests (at most kSmallEvals evaluations, about
+// a quarter of a millisecond of GPU time, e.g. one 1-CDP panel), so
+// interactive panels never queue behind a longer search.
+constexpr double kSmallEvals = 5e8;
 // Device bytes each slot keeps cached between requests (outside the budget).
 constexpr size_t kSlotCache = size_t(512) << 20;
 
@@ -137,6 +138,27 @@ struct sb_server {
   std::vector<std::thread> workers;
   std::vector<sb_ctx*> ctx;
   sb_ctx* res_ctx = nullptr;  // owns the resident gathers
+  std::vector<void*> pin;     // per slot: pinned download staging
+  std::vector<size_t> pin_cap;
+
+  // Downloads n = rows * ns results of `res` through the slot's pinned
+  // buffer (a true DMA; pageable copies serialise through the runtime's
+  // staging) and hands out the six arrays.
+  int download(int slot, const sb_result* res, size_t n, sb_host_result* h) {
+    const size_t bytes = n * 24;
+    if (bytes > pin_cap[slot]) {
+      if (pin[slot]) (void)hipHostFree(pin[slot]);
+      pin[slot] = nullptr;
+      pin_cap[slot] = 0;
+      if (hipHostMalloc(&pin[slot], bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(SB_ERR_HIP, "pinned staging allocation failed");
+      pin_cap[slot] = bytes;
+    }
+    char* p = (char*)pin[slot];
+    *h = sb_host_result{(int32_t*)p, (float*)(p + n * 4), (float*)(p + n * 8), (float*)(p + n * 12),
+                        (float*)(p + n * 16), (float*)(p + n * 20)};
+    return sb_result_download(ctx[slot], res, h);
+  }
   std::map<int, Resident> resident;
   int next_gid = 1;
 
@@ -175,20 +197,17 @@ int sb_server::process_range(int slot, std::vector<std::shared_ptr<Req>>& batch)
     o += ce - cb;
   }
   if (!rc) {
-    const size_t n = (size_t)rows * r0.ns;
-    std::vector<int32_t> idx(n);
-    std::vector<float> a(n), b(n), cc(n), coh(n), stk(n);
-    sb_host_result h{idx.data(), a.data(), b.data(), cc.data(), coh.data(), stk.data()};
-    rc = sb_result_download(c, res, &h);
+    sb_host_result h;
+    rc = download(slot, res, (size_t)rows * r0.ns, &h);
     size_t k = 0;
     for (auto& q : batch) {
       const size_t m = (size_t)(q->r.cdp_end - q->r.cdp_begin) * r0.ns;
-      q->index.assign(idx.begin() + k, idx.begin() + k + m);
-      q->a.assign(a.begin() + k, a.begin() + k + m);
-      q->b.assign(b.begin() + k, b.begin() + k + m);
-      q->c.assign(cc.begin() + k, cc.begin() + k + m);
-      q->coher.assign(coh.begin() + k, coh.begin() + k + m);
-      q->stack.assign(stk.begin() + k, stk.begin() + k + m);
+      q->index.assign(h.index + k, h.index + k + m);
+      q->a.assign(h.a + k, h.a + k + m);
+      q->b.assign(h.b + k, h.b + k + m);
+      q->c.assign(h.c + k, h.c + k + m);
+      q->coher.assign(h.coher + k, h.coher + k + m);
+      q->stack.assign(h.stack + k, h.stack + k + m);
       k += m;
     }
   }
@@ -250,20 +269,17 @@ int sb_server::process(int slot, std::vector<std::shared_ptr<Req>>& batch) {
   rc = sb_result_create(c, ncdp, r0.ns, &res);
   if (!rc) rc = r0.method == SB_METHOD_CMP ? sb_cmp_run(c, g, &r0.cmp, res) : sb_crs_run(c, g, &r0.crs, res);
   if (!rc) {
-    const size_t n = (size_t)ncdp * r0.ns;
-    std::vector<int32_t> idx(n);
-    std::vector<float> a(n), b(n), cc(n), coh(n), stk(n);
-    sb_host_result h{idx.data(), a.data(), b.data(), cc.data(), coh.data(), stk.data()};
-    rc = sb_result_download(c, res, &h);
+    sb_host_result h;
+    rc = download(slot, res, (size_t)ncdp * r0.ns, &h);
     size_t o = 0;
     for (auto& q : batch) {
       const size_t m = (size_t)q->r.ncdp * r0.ns;
-      q->index.assign(idx.begin() + o, idx.begin() + o + m);
-      q->a.assign(a.begin() + o, a.begin() + o + m);
-      q->b.assign(b.begin() + o, b.begin() + o + m);
-      q->c.assign(cc.begin() + o, cc.begin() + o + m);
-      q->coher.assign(coh.begin() + o, coh.begin() + o + m);
-      q->stack.assign(stk.begin() + o, stk.begin() + o + m);
+      q->index.assign(h.index + o, h.index + o + m);
+      q->a.assign(h.a + o, h.a + o + m);
+      q->b.assign(h.b + o, h.b + o + m);
+      q->c.assign(h.c + o, h.c + o + m);
+      q->coher.assign(h.coher + o, h.coher + o + m);
+      q->stack.assign(h.stack + o, h.stack + o + m);
       o += m;
     }
   }
@@ -365,6 +381,8 @@ extern "C" int sb_server_create(int device, size_t hbm_budget, int max_batch, sb
       return rc;
     }
   }
+  s->pin.assign(kSlots, nullptr);
+  s->pin_cap.assign(kSlots, 0);
   // Every slot ctx may use the whole budget; the server keeps the sum within it.
   size_t b = 0;
   sb_ctx_memory(s->ctx[0], nullptr, &b);
@@ -553,5 +571,7 @@ extern "C" void sb_server_destroy(sb_server* s) {
   s->resident.clear();
   for (auto* c : s->ctx) sb_ctx_destroy(c);
   sb_ctx_destroy(s->res_ctx);
+  for (void* p : s->pin)
+    if (p) (void)hipHostFree(p);
   delete s;
 }
