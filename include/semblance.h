@@ -245,7 +245,9 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
 /* Context options.  SB_OPT_PERSISTENT = 1: the global-memory fast search
  * runs as a persistent grid (as many workgroups as the CUs hold at once,
  * each XCD's workgroups pulling that XCD's work items from their own
- * counter) instead of one workgroup per work item.  Results are the same. */
+ * counter) instead of one workgroup per work item.  Results are the same.
+ * No reference counterpart (the reference launches one grid per CDP,
+ * CMP/CUDA/src/main.cu:271-304). */
 enum { SB_OPT_PERSISTENT = 1 };
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);
