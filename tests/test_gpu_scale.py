@@ -192,3 +192,63 @@ def test_crs_fast_larger_gather(sb, orc, gpu_ctx):
             sa = ref["coher"][cdp, t0]
             sb_ = out["coher"][cdp, t0]
             assert abs(sa - sb_) <= 1e-5, (cdp, t0, sa, sb_)
+
+
+def _crs_line(sb, ncdp=6):
+    cdp_off, geom, samples = sb.synth(1, ncdp, (60, 120), 501, 4000, SEED + 7, h_max=1500.0, mid_half_width=40.0,
+                                      nevents=6, snr=3.0)
+    nbr_off, nbr = sb.neighbors(geom[cdp_off[:-1]], 40)
+    return cdp_off, geom, samples, nbr_off, nbr
+
+
+def _crs_errors(orc, out, line, cells):
+    cdp_off, geom, samples, nbr_off, nbr = line
+    rel, stk = [], []
+    scale = float(np.max(np.abs(samples)))
+    for cdp, t0 in cells:
+        sem64, stk64 = orc.crs_eval_f64(int(cdp), cdp_off, geom, samples, 4000, nbr_off, nbr, float(out["a"][cdp, t0]),
+                                        float(out["b"][cdp, t0]), float(out["c"][cdp, t0]), 0.004, int(t0))
+        if sem64 >= 0.05:
+            rel.append(abs(out["coher"][cdp, t0] - sem64) / sem64)
+            stk.append(abs(out["stack"][cdp, t0] - stk64) / scale)
+    return np.array(rel), np.array(stk)
+
+
+@pytest.mark.parametrize("store,tol", [("f16", 2e-3), ("bf16", 2e-2)])
+def test_crs_reduced_precision_store_error_vs_fp64(sb, orc, gpu_ctx, store, tol):
+    """CRS grid search on a 16-bit store: the reported semblance and stack at
+    the reported (A, B, C) against the FP64 evaluation of the same point
+    (tolerance as for CMP: 2e-3 relative semblance for FP16, stack relative to
+    the line's amplitude scale)."""
+    print("case: crs store", store, flush=True)
+    line = _crs_line(sb)
+    cdp_off, geom, samples, nbr_off, nbr = line
+    g = sb.Gather(gpu_ctx, cdp_off, geom, samples, 4000, store=sb.STORES[store], nbr_off=nbr_off, nbr=nbr)
+    out = sb.search(gpu_ctx, g, sb.crs_params(BOX[0], BOX[1], 7, BOX[2], BOX[3], 5, BOX[4], BOX[5], 9, 0.004))
+    g.close()
+    rng = np.random.default_rng(1)
+    cells = list(zip(rng.integers(0, 6, 120), rng.integers(20, 480, 120)))
+    rel, stk = _crs_errors(orc, out, line, cells)
+    print("crs store %s: semblance rel err vs FP64 max %.3g p99 %.3g (n=%d); stack err max %.3g" %
+          (store, rel.max(), np.percentile(rel, 99), len(rel), stk.max()))
+    assert len(rel) > 30
+    assert rel.max() <= tol and stk.max() <= tol
+
+
+@pytest.mark.parametrize("store,tol", [("f32", 2e-4), ("f16", 2e-3)])
+def test_crs_de_fitness_error_vs_fp64(sb, orc, gpu_ctx, store, tol):
+    """CRS-DE (fast evaluator, FP32 accumulation): the reported fitness and
+    stack against the FP64 evaluation of the reported (A, B, C)."""
+    print("case: crs-de store", store, flush=True)
+    line = _crs_line(sb)
+    cdp_off, geom, samples, nbr_off, nbr = line
+    g = sb.Gather(gpu_ctx, cdp_off, geom, samples, 4000, store=sb.STORES[store], nbr_off=nbr_off, nbr=nbr)
+    out = sb.search(gpu_ctx, g, sb.de_params(*BOX, 0.004, pop=32, ngen=12, seed=9))
+    g.close()
+    rng = np.random.default_rng(2)
+    cells = list(zip(rng.integers(0, 6, 120), rng.integers(20, 480, 120)))
+    rel, stk = _crs_errors(orc, out, line, cells)
+    print("crs-de store %s: fitness rel err vs FP64 max %.3g p99 %.3g (n=%d); stack err max %.3g" %
+          (store, rel.max(), np.percentile(rel, 99), len(rel), stk.max()))
+    assert len(rel) > 30
+    assert rel.max() <= tol and stk.max() <= tol
