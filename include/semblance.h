@@ -248,9 +248,25 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
  * counter) instead of one workgroup per work item.  Results are the same.
  * No reference counterpart (the reference launches one grid per CDP,
  * CMP/CUDA/src/main.cu:271-304). */
-enum { SB_OPT_PERSISTENT = 1 };
+enum { SB_OPT_PERSISTENT = 1, SB_OPT_STREAM_ROWS = 2 /* traces per sb_search_streamed batch, 0 = auto */ };
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);
+
+/* ------------------------------------------------------ streamed search */
+/* One-shot search of a line whose traces are still in host memory, with the
+ * upload overlapped: the traces go up in CDP batches on a copy stream
+ * (staged, padded or quantised, slot tables built) while each batch's CDPs
+ * -- for CRS / CRS-DE, each CDP once all its neighbour CDPs have arrived --
+ * are searched on the ctx stream.  kind: SB_METHOD_CMP (sb_cmp_params),
+ * SB_METHOD_CRS (sb_crs_params) or SB_GRAPH_CRS_DE (sb_de_params); nbr_off /
+ * nbr as for sb_gather_create (ignored for CMP).  out receives ncdp x ns
+ * results, the same values as sb_gather_create followed by one search.
+ * keep != NULL receives the resident gather for further searches.  Replaces
+ * the reference's read-then-compute sequence (CMP/OpenMP/src/main.cpp:77-236,
+ * CRS/OpenMP/src/main.cpp:98-317). */
+int sb_search_streamed(sb_ctx* ctx, int kind, const void* params, int ncdp, const int32_t* cdp_off, int ns,
+                       int dt_us, const sb_trace_geom* geom, const float* samples, int store,
+                       const int32_t* nbr_off, const int32_t* nbr, sb_host_result* out, sb_gather** keep);
 
 /* --------------------------------------------------------------- graphs */
 /* One search captured into a HIP graph and replayed without host work:

@@ -16,6 +16,7 @@ METHOD_CMP, METHOD_CRS = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 MODE_FAST, MODE_EXACT = 0, 1
 OPT_PERSISTENT = 1
+OPT_STREAM_ROWS = 2
 GRAPH_CRS_DE = 2
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
 
@@ -94,6 +95,7 @@ EXPORTS = [
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
     "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
+    "sb_search_streamed",
 ]
 
 
@@ -167,6 +169,8 @@ def lib():
         L.sb_ctx_get_option.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
         L.sb_graph_create.argtypes = [P, P, C.c_int, P, C.c_int, C.c_int, P, C.POINTER(P)]
         L.sb_graph_launch.argtypes = [P]
+        L.sb_search_streamed.argtypes = [P, C.c_int, P, C.c_int, i32p, C.c_int, C.c_int, P, C.POINTER(C.c_float),
+                                         C.c_int, i32p, i32p, C.POINTER(HostResult), C.POINTER(P)]
         L.sb_graph_destroy.argtypes = [P]
         L.sb_graph_destroy.restype = None
         _lib = L
@@ -325,6 +329,13 @@ class Gather:
                                      _ptr(no, C.c_int32) if no is not None else None,
                                      _ptr(nb, C.c_int32) if nb is not None else None, C.byref(self.h)))
 
+    @classmethod
+    def adopt(cls, ctx, handle, cdp_off, geom, ncdp, ns):
+        """Wraps a gather the library created (sb_search_streamed's keep)."""
+        self = cls.__new__(cls)
+        self.ctx, self.cdp_off, self.geom, self.ncdp, self.ns, self.h = ctx, cdp_off, geom, ncdp, ns, handle
+        return self
+
     def info(self):
         v = [C.c_int() for _ in range(4)]
         b = C.c_size_t()
@@ -389,6 +400,34 @@ def search(ctx, gather, params, cdp_range=None):
         return r.download()
     finally:
         r.close()
+
+
+def search_streamed(ctx, params, cdp_off, geom, samples, dt_us, store=STORE_F32, nbr_off=None, nbr=None,
+                    keep=False):
+    """sb_search_streamed: uploads the line in CDP batches overlapped with the
+    search; returns the results (and the resident Gather when keep=True)."""
+    off = _i32(cdp_off)
+    g = np.ascontiguousarray(geom, GEOM_DTYPE)
+    s = np.ascontiguousarray(samples, np.float32)
+    ncdp, ns = len(off) - 1, s.shape[1]
+    kind = METHOD_CMP if isinstance(params, CmpParams) else METHOD_CRS if isinstance(params, CrsParams) \
+        else GRAPH_CRS_DE
+    no = nb = None
+    if nbr_off is not None:
+        no, nb = _i32(nbr_off), _i32(nbr if len(nbr) else [0])
+    out = {k: np.zeros((ncdp, ns), np.int32 if k == "index" else np.float32)
+           for k in ("index", "a", "b", "c", "coher", "stack")}
+    hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float)
+                      for k in ("index", "a", "b", "c", "coher", "stack")])
+    gh = C.c_void_p()
+    check(lib().sb_search_streamed(ctx.h, kind, C.cast(C.pointer(params), C.c_void_p), ncdp, _ptr(off, C.c_int32), ns,
+                                   dt_us, g.ctypes.data, _ptr(s, C.c_float), store,
+                                   None if no is None else _ptr(no, C.c_int32),
+                                   None if nb is None else _ptr(nb, C.c_int32), C.byref(hr),
+                                   C.byref(gh) if keep else None))
+    if not keep:
+        return out
+    return out, Gather.adopt(ctx, gh, off, g, ncdp, ns)
 
 
 class Graph:

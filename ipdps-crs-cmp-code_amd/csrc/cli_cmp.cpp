@@ -27,11 +27,11 @@ int main(int argc, const char** argv) {
 
   Session s;
   s.load(path, SB_METHOD_CMP, aph, 0);
+  s.open(env_device(0));
   const double t0 = now_s();
-  s.upload(env_device(0), false);
   sb_cmp_params prm{c0, c1, nc, tau, env_mode()};
   HostOut out((size_t)s.ncdp * s.ns);
-  check(sb_cmp(s.ctx, s.g, &prm, &out.view), "CMP search");
+  check(s.search(SB_METHOD_CMP, &prm, &out.view), "CMP search");
   const double t = now_s() - t0;
   const double stps = ((double)s.ntr / 1e9) * s.ns * nc / t;
   log(INFO, "Execution Time: " + std::to_string(t) + ", Semblances-Trace/s: " + std::to_string(stps));

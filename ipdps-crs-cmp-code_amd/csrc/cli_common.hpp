@@ -124,7 +124,7 @@ inline int env_device(int fallback) {
   return d ? std::atoi(d) : fallback;
 }
 
-// Loads the file, creates the device gather; the caller owns both.
+// Loads the file and opens the device; the search streams the traces up.
 struct Session {
   sb_su_gather* su = nullptr;
   sb_ctx* ctx = nullptr;
@@ -139,11 +139,13 @@ struct Session {
     check(sb_su_load(path.c_str(), method, aph, apm, &su), "reading input");
     check(sb_su_info(su, &ncdp, &ntr, &ns, &dt_us, &nnbr), "input");
   }
-  void upload(int device, bool crs) {
-    check(sb_ctx_create(device, 0, &ctx), "device");
-    check(sb_gather_create(ctx, ncdp, sb_su_cdp_offsets(su), ns, dt_us, sb_su_geom(su), sb_su_samples(su),
-                           env_store(), crs ? sb_su_nbr_offsets(su) : nullptr, crs ? sb_su_nbr(su) : nullptr, &g),
-          "upload");
+  void open(int device) { check(sb_ctx_create(device, 0, &ctx), "device"); }
+  // Upload overlapped with the search (sb_search_streamed).
+  int search(int kind, const void* params, sb_host_result* out) {
+    const bool crs = kind != SB_METHOD_CMP;
+    return sb_search_streamed(ctx, kind, params, ncdp, sb_su_cdp_offsets(su), ns, dt_us, sb_su_geom(su),
+                              sb_su_samples(su), env_store(), crs ? sb_su_nbr_offsets(su) : nullptr,
+                              crs ? sb_su_nbr(su) : nullptr, out, nullptr);
   }
   // Sum over CDPs of the traces entering each evaluation (CMP: fold; CRS: supergather).
   double traces_per_eval(bool crs) const {

@@ -43,10 +43,10 @@ int main(int argc, const char** argv) {
 
   Session s;
   s.load(path, SB_METHOD_CRS, aph, apm);
+  s.open(env_device(0));
   const double t0 = now_s();
-  s.upload(env_device(0), true);
   HostOut out((size_t)s.ncdp * s.ns);
-  check(sb_crs(s.ctx, s.g, &prm, &out.view), "CRS search");
+  check(s.search(SB_METHOD_CRS, &prm, &out.view), "CRS search");
   const double t = now_s() - t0;
   const double npar = (double)prm.na * prm.nb * prm.nc;
   const double stps = (s.traces_per_eval(true) / 1e9) * (s.ns * npar / t);

@@ -63,10 +63,10 @@ int main(int argc, const char** argv) {
 
   Session s;
   s.load(path, SB_METHOD_CRS, aph, apm);
+  s.open(env_device(d > 0 ? d - 1 : 0));
   const double t0 = now_s();
-  s.upload(env_device(d > 0 ? d - 1 : 0), true);
   HostOut out((size_t)s.ncdp * s.ns);
-  check(sb_crs_de(s.ctx, s.g, &prm, &out.view), "CRS-DE search");
+  check(s.search(SB_GRAPH_CRS_DE, &prm, &out.view), "CRS-DE search");
   const double t = now_s() - t0;
   const double evals = (double)prm.pop * (prm.ngen + 1);
   const double stps = (s.traces_per_eval(true) / 1e9) * (s.ns * evals / t);

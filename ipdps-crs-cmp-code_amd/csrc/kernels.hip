@@ -67,10 +67,12 @@ __global__ void k_prep_traces(const TraceGeom* __restrict__ g, int ntr, float* _
 }
 
 // CMP slot table: slot k of CDP i is row cdp_off[i] + k, H = h * idt^2.
-__global__ void k_cmp_slots(int ntr, const double* __restrict__ h0, const float* __restrict__ scale, double idt,
-                            SlotF* __restrict__ slots) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ntr) return;
+// Rows [row0, row0 + n).
+__global__ void k_cmp_slots(int row0, int n, const double* __restrict__ h0, const float* __restrict__ scale,
+                            double idt, SlotF* __restrict__ slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = row0 + i;
   SlotF s;
   s.row = r;
   s.m = 0.f;
@@ -82,16 +84,16 @@ __global__ void k_cmp_slots(int ntr, const double* __restrict__ h0, const float*
   slots[r] = s;
 }
 
-// CRS supergather slot tables, one block per CDP: neighbour CDPs in list
-// order, traces in gather order; distances to the CDP's first trace
-// (CRS/OpenMP/src/main.cpp:179-198, 287-311).
-__global__ void k_crs_slots(const int32_t* __restrict__ cdp_off, const int32_t* __restrict__ nbr_off,
+// CRS supergather slot tables, one block per CDP (CDPs cdp0 + blockIdx.x):
+// neighbour CDPs in list order, traces in gather order; distances to the
+// CDP's first trace (CRS/OpenMP/src/main.cpp:179-198, 287-311).
+__global__ void k_crs_slots(int cdp0, const int32_t* __restrict__ cdp_off, const int32_t* __restrict__ nbr_off,
                             const int32_t* __restrict__ nbr, const int32_t* __restrict__ slot_off,
                             const double* __restrict__ m0x, const double* __restrict__ m0y,
                             const double* __restrict__ h0, const float* __restrict__ scale, double idt,
                             SlotF* __restrict__ sf, SlotD* __restrict__ sd, SlotF* __restrict__ sp) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x;
+  const int i = cdp0 + blockIdx.x;
   const int ref = cdp_off[i];
   const double rx = m0x[ref], ry = m0y[ref];
   int base = slot_off[i];
@@ -676,19 +678,19 @@ hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, doubl
   return SB_CHECK_LAUNCH();
 }
 
-hipError_t launch_cmp_slots(int ntr, const double* h0, const float* scale, double idt, SlotF* slots,
+hipError_t launch_cmp_slots(int row0, int n, const double* h0, const float* scale, double idt, SlotF* slots,
                             hipStream_t st) {
-  if (ntr <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cmp_slots, dim3((ntr + 255) / 256), dim3(256), 0, st, ntr, h0, scale, idt, slots);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cmp_slots, dim3((n + 255) / 256), dim3(256), 0, st, row0, n, h0, scale, idt, slots);
   return SB_CHECK_LAUNCH();
 }
 
-hipError_t launch_crs_slots(int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
+hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
                             const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
                             const float* scale, double idt, SlotF* sf, SlotD* sd, SlotF* sp, hipStream_t st) {
   if (ncdp <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_crs_slots, dim3(ncdp), dim3(256), 0, st, cdp_off, nbr_off, nbr, slot_off, m0x, m0y, h0, scale,
-                     idt, sf, sd, sp);
+  hipLaunchKernelGGL(k_crs_slots, dim3(ncdp), dim3(256), 0, st, cdp0, cdp_off, nbr_off, nbr, slot_off, m0x, m0y, h0,
+                     scale, idt, sf, sd, sp);
   return SB_CHECK_LAUNCH();
 }
 

@@ -213,14 +213,52 @@ int upload(sb_ctx* c, void* dst, const void* src, size_t bytes) {
 
 }  // namespace
 
-extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
-                                const sb_trace_geom* geom, const float* samples, int store, const int32_t* nbr_off,
-                                const int32_t* nbr, sb_gather** out) {
-  if (!ctx || !out || !cdp_off || ncdp <= 0 || ns <= 0 || dt_us <= 0 || !geom || !samples)
-    return fail(SB_ERR_INVALID, "sb_gather_create: bad arguments");
+namespace sb {
+
+// Incremental construction of a gather: begin() validates, allocates and
+// prepares the geometry (and the CRS supergather offsets); batch() uploads
+// the samples of a CDP range on stream `cs` -- staged, padded or quantised --
+// with their CMP slots, then the CRS slot tables of every CDP whose
+// neighbours have all arrived, and records an event; finish() waits and
+// hands the gather over.  sb_gather_create runs the batches back to back;
+// sb_search_streamed overlaps them with range searches.
+struct GatherBuild {
+  sb_ctx* ctx = nullptr;
+  std::unique_ptr<sb_gather, void (*)(sb_gather*)> G{nullptr, sb_gather_destroy};
+  const float* samples = nullptr;
+  hipStream_t cs = nullptr;
+  void* stage[2] = {nullptr, nullptr};
+  int64_t stage_rows = 0;
+  int which = 0;
+  void *dnoff = nullptr, *dnbr = nullptr;  // CRS neighbour lists on the device
+  std::vector<int32_t> nbr_reach;          // max neighbour index over CDPs [0, i]
+  int crs_ready = 0;                       // CDPs [0, crs_ready) have slot tables
+  double idt = 0;
+
+  ~GatherBuild() {
+    if (cs) (void)hipStreamSynchronize(cs);  // staged copies done before the buffers are reused
+    for (void* p : stage) ctx_free(ctx, p);
+    ctx_free(ctx, dnoff);
+    ctx_free(ctx, dnbr);
+  }
+
+  int begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int dt_us, const sb_trace_geom* geom,
+            const float* smp, int store, const int32_t* nbr_off, const int32_t* nbr, hipStream_t copy_stream,
+            int64_t batch_rows);
+  int batch(int cb, int ce, hipEvent_t ev, int* ready);
+  int finish(sb_gather** out);
+};
+
+int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int dt_us, const sb_trace_geom* geom,
+                       const float* smp, int store, const int32_t* nbr_off, const int32_t* nbr,
+                       hipStream_t copy_stream, int64_t batch_rows) {
+  ctx = c;
+  cs = copy_stream;
+  samples = smp;
+  if (!ctx || !cdp_off || ncdp <= 0 || ns <= 0 || dt_us <= 0 || !geom || !samples)
+    return fail(SB_ERR_INVALID, "gather: bad arguments");
   if (store != SB_STORE_F32 && store != SB_STORE_F16 && store != SB_STORE_BF16)
-    return fail(SB_ERR_INVALID, "sb_gather_create: bad store type");
-  *out = nullptr;
+    return fail(SB_ERR_INVALID, "gather: bad store type");
   if (cdp_off[0] != 0) return fail(SB_ERR_INVALID, "cdp_off[0] must be 0");
   for (int i = 0; i < ncdp; i++)
     if (cdp_off[i + 1] <= cdp_off[i]) return fail(SB_ERR_INVALID, "every CDP needs at least one trace");
@@ -235,8 +273,7 @@ extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, i
       if (nbr[q] < 0 || nbr[q] >= ncdp) return fail(SB_ERR_INVALID, "neighbour index out of range");
   }
   HIPR(hipSetDevice(ctx->device));
-
-  std::unique_ptr<sb_gather, void (*)(sb_gather*)> G(new sb_gather, sb_gather_destroy);
+  G.reset(new sb_gather);
   G->ctx = ctx;
   G->serial = ++ctx->gather_serial;
   G->ncdp = ncdp;
@@ -248,6 +285,7 @@ extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, i
   // read one dword past the last sample.
   G->stride = ((int64_t)ns + 8 + 63) / 64 * 64;
   G->cdp_off.assign(cdp_off, cdp_off + ncdp + 1);
+  G->cmp_hmax = INFINITY;  // known once every slot exists (finish)
   const size_t esz = store == SB_STORE_F32 ? 4 : 2;
   int rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * G->stride * esz, &G->d_samples))) return rc;
@@ -258,101 +296,131 @@ extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, i
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0x))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0y))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(SlotF), (void**)&G->d_cmp_slots))) return rc;
-  if ((rc = upload(ctx, G->d_cdp_off, cdp_off, (size_t)(ncdp + 1) * 4))) return rc;
+  stage_rows = std::max<int64_t>(1, std::min<int64_t>(batch_rows, ntr));
+  for (auto& st : stage)
+    if ((rc = ctx_alloc(ctx, (size_t)stage_rows * ns * 4, &st))) return rc;
+  idt = 1.0 / ((double)dt_us / 1e6);
 
-  // Samples: staged in chunks of rows, then padded (F32) or quantised (F16/BF16).
-  {
-    const int64_t chunk_rows = std::max<int64_t>(1, (int64_t)(64 << 20) / ((int64_t)ns * 4));
-    const int64_t rows = std::min<int64_t>(chunk_rows, ntr);
-    void* stage = nullptr;
-    if ((rc = ctx_alloc(ctx, (size_t)rows * ns * 4, &stage))) return rc;
-    for (int64_t r0 = 0; r0 < ntr; r0 += rows) {
-      const int64_t nr = std::min<int64_t>(rows, ntr - r0);
-      hipError_t e = hipMemcpyAsync(stage, samples + r0 * ns, (size_t)nr * ns * 4, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess)
-        e = launch_store((const float*)stage, (int)nr, ns, G->stride, store,
-                         (char*)G->d_samples + (size_t)r0 * G->stride * esz, G->d_scale ? G->d_scale + r0 : nullptr,
-                         ctx->stream);
-      if (e != hipSuccess) {
-        ctx_free(ctx, stage);
-        return hip_fail(e, "sample upload");
-      }
-    }
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    ctx_free(ctx, stage);
-    if (e != hipSuccess) return hip_fail(e, "sample upload");
-  }
-
-  // Geometry: FP64 (and the CMP float reference arithmetic) on the device.
+  // Geometry (FP64, and the CMP float reference arithmetic) on the device,
+  // on the copy stream ahead of every batch.
+  HIPR(hipMemcpyAsync(G->d_cdp_off, cdp_off, (size_t)(ncdp + 1) * 4, hipMemcpyHostToDevice, cs));
   {
     std::vector<TraceGeom> tg(ntr);
     for (int i = 0; i < ntr; i++)
       tg[i] = TraceGeom{geom[i].cdp, geom[i].scalco, geom[i].sx, geom[i].sy, geom[i].gx, geom[i].gy};
     void* dg = nullptr;
     if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(TraceGeom), &dg))) return rc;
-    hipError_t e = hipMemcpyAsync(dg, tg.data(), (size_t)ntr * sizeof(TraceGeom), hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(dg, tg.data(), (size_t)ntr * sizeof(TraceGeom), hipMemcpyHostToDevice, cs);
     if (e == hipSuccess)
-      e = launch_prep_traces((const TraceGeom*)dg, ntr, G->d_h_ref32, G->d_h0, G->d_m0x, G->d_m0y, ctx->stream);
-    const double idt = 1.0 / ((double)dt_us / 1e6);
-    if (e == hipSuccess) e = launch_cmp_slots(ntr, G->d_h0, G->d_scale, idt, G->d_cmp_slots, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      e = launch_prep_traces((const TraceGeom*)dg, ntr, G->d_h_ref32, G->d_h0, G->d_m0x, G->d_m0y, cs);
+    // tg is read by the copy: wait before it goes out of scope.
+    if (e == hipSuccess) e = hipStreamSynchronize(cs);
     ctx_free(ctx, dg);
     if (e != hipSuccess) return hip_fail(e, "geometry");
-    // Largest scaled h: bounds the LDS slabs of the CMP search.
-    std::vector<SlotF> hs(ntr);
-    HIPR(hipMemcpy(hs.data(), G->d_cmp_slots, (size_t)ntr * sizeof(SlotF), hipMemcpyDeviceToHost));
-    G->cmp_hmax = 0.f;
-    for (int i = 0; i < ntr; i++) G->cmp_hmax = std::max(G->cmp_hmax, hs[i].h);
   }
 
-  // CRS supergathers.
+  // CRS supergathers: offsets on the host; the tables follow the samples.
   if (nbr_off) {
     G->has_crs = true;
     G->crs_slot_off.resize(ncdp + 1);
     G->crs_slot_off[0] = 0;
+    nbr_reach.resize(ncdp);
+    int32_t reach = 0;
     for (int i = 0; i < ncdp; i++) {
-      int64_t s = 0;
-      for (int q = nbr_off[i]; q < nbr_off[i + 1]; q++) s += cdp_off[nbr[q] + 1] - cdp_off[nbr[q]];
-      if (G->crs_slot_off[i] + s > INT32_MAX) return fail(SB_ERR_INVALID, "supergathers too large");
-      G->crs_slot_off[i + 1] = G->crs_slot_off[i] + (int32_t)s;
+      int64_t sz = 0;
+      for (int q = nbr_off[i]; q < nbr_off[i + 1]; q++) {
+        sz += cdp_off[nbr[q] + 1] - cdp_off[nbr[q]];
+        reach = std::max(reach, nbr[q]);
+      }
+      nbr_reach[i] = reach;
+      if (G->crs_slot_off[i] + sz > INT32_MAX) return fail(SB_ERR_INVALID, "supergathers too large");
+      G->crs_slot_off[i + 1] = G->crs_slot_off[i] + (int32_t)sz;
     }
     G->ncrs_slots = G->crs_slot_off[ncdp];
-    void *dnoff = nullptr, *dnbr = nullptr;
     if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, (void**)&G->d_crs_slot_off))) return rc;
     if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotF), (void**)&G->d_crs_slots))) return rc;
     if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotD), (void**)&G->d_crs_slots_d))) return rc;
     if ((rc = ctx_alloc(ctx, (size_t)G->ncrs_slots * sizeof(SlotF), (void**)&G->d_crs_slots_p))) return rc;
     if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, &dnoff))) return rc;
-    if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(nnbr, 1) * 4, &dnbr))) {
-      ctx_free(ctx, dnoff);
-      return rc;
-    }
-    hipError_t e = hipMemcpyAsync(G->d_crs_slot_off, G->crs_slot_off.data(), (size_t)(ncdp + 1) * 4,
-                                  hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dnoff, nbr_off, (size_t)(ncdp + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && nnbr) e = hipMemcpyAsync(dnbr, nbr, (size_t)nnbr * 4, hipMemcpyHostToDevice, ctx->stream);
-    const double idt = 1.0 / ((double)dt_us / 1e6);
-    if (e == hipSuccess)
-      e = launch_crs_slots(ncdp, G->d_cdp_off, (const int32_t*)dnoff, (const int32_t*)dnbr, G->d_crs_slot_off,
-                           G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_crs_slots, G->d_crs_slots_d,
-                           G->d_crs_slots_p, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    ctx_free(ctx, dnoff);
-    ctx_free(ctx, dnbr);
-    if (e != hipSuccess) return hip_fail(e, "supergather tables");
+    if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(nnbr, 1) * 4, &dnbr))) return rc;
+    HIPR(hipMemcpyAsync(G->d_crs_slot_off, G->crs_slot_off.data(), (size_t)(ncdp + 1) * 4, hipMemcpyHostToDevice,
+                        cs));
+    HIPR(hipMemcpyAsync(dnoff, nbr_off, (size_t)(ncdp + 1) * 4, hipMemcpyHostToDevice, cs));
+    if (nnbr) HIPR(hipMemcpyAsync(dnbr, nbr, (size_t)nnbr * 4, hipMemcpyHostToDevice, cs));
+    // crs_slot_off (a member) outlives the copies; the caller's lists are
+    // read before the first batch's work.
+    HIPR(hipStreamSynchronize(cs));
   }
-  G->bytes = 0;
+  return SB_OK;
+}
+
+// Samples and slots of CDPs [cb, ce) (rows of at most stage_rows per copy).
+// *ready = the number of leading CDPs searchable once `ev` has fired.
+int GatherBuild::batch(int cb, int ce, hipEvent_t ev, int* ready) {
+  const int ns = G->ns, store = G->store;
+  const size_t esz = store == SB_STORE_F32 ? 4 : 2;
+  const int64_t rb = G->cdp_off[cb], re = G->cdp_off[ce];
+  for (int64_t r0 = rb; r0 < re; r0 += stage_rows) {
+    const int64_t nr = std::min<int64_t>(stage_rows, re - r0);
+    void* st = stage[which];
+    which ^= 1;  // the next copy fills the other buffer while this one is converted
+    HIPR(hipMemcpyAsync(st, samples + r0 * ns, (size_t)nr * ns * 4, hipMemcpyHostToDevice, cs));
+    HIPR(launch_store((const float*)st, (int)nr, ns, G->stride, store, (char*)G->d_samples + (size_t)r0 * G->stride * esz,
+                      G->d_scale ? G->d_scale + r0 : nullptr, cs));
+  }
+  HIPR(launch_cmp_slots((int)rb, (int)(re - rb), G->d_h0, G->d_scale, idt, G->d_cmp_slots, cs));
+  int done = ce;
+  if (G->has_crs) {
+    // CDPs whose every neighbour lies below ce (the reach is a prefix maximum).
+    int r = crs_ready;
+    while (r < G->ncdp && nbr_reach[r] < ce) r++;
+    HIPR(launch_crs_slots(crs_ready, r - crs_ready, G->d_cdp_off, (const int32_t*)dnoff, (const int32_t*)dnbr,
+                          G->d_crs_slot_off, G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_crs_slots,
+                          G->d_crs_slots_d, G->d_crs_slots_p, cs));
+    crs_ready = r;
+    done = r;
+  }
+  if (ev) HIPR(hipEventRecord(ev, cs));
+  *ready = done;
+  return SB_OK;
+}
+
+int GatherBuild::finish(sb_gather** out) {
+  HIPR(hipStreamSynchronize(cs));
+  sb_gather* g = G.get();
+  // Largest scaled h: bounds the LDS slabs of the CMP search.
+  std::vector<SlotF> hs(g->ntr);
+  HIPR(hipMemcpy(hs.data(), g->d_cmp_slots, (size_t)g->ntr * sizeof(SlotF), hipMemcpyDeviceToHost));
+  g->cmp_hmax = 0.f;
+  for (int i = 0; i < g->ntr; i++) g->cmp_hmax = std::max(g->cmp_hmax, hs[i].h);
+  g->bytes = 0;
   {
     std::lock_guard<std::mutex> lk(ctx->mem_mu);
-    for (void* p : {(void*)G->d_samples, (void*)G->d_scale, (void*)G->d_cdp_off, (void*)G->d_h_ref32, (void*)G->d_h0,
-                    (void*)G->d_m0x, (void*)G->d_m0y, (void*)G->d_cmp_slots, (void*)G->d_crs_slot_off,
-                    (void*)G->d_crs_slots, (void*)G->d_crs_slots_d, (void*)G->d_crs_slots_p}) {
+    for (void* p : {(void*)g->d_samples, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
+                    (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
+                    (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p}) {
       auto it = ctx->sizes.find(p);
-      if (it != ctx->sizes.end()) G->bytes += it->second;
+      if (it != ctx->sizes.end()) g->bytes += it->second;
     }
   }
   *out = G.release();
   return SB_OK;
+}
+
+}  // namespace sb
+
+extern "C" int sb_gather_create(sb_ctx* ctx, int ncdp, const int32_t* cdp_off, int ns, int dt_us,
+                                const sb_trace_geom* geom, const float* samples, int store, const int32_t* nbr_off,
+                                const int32_t* nbr, sb_gather** out) {
+  if (!ctx || !out) return fail(SB_ERR_INVALID, "sb_gather_create: bad arguments");
+  *out = nullptr;
+  GatherBuild b;
+  const int64_t rows = std::max<int64_t>(1, (int64_t)(64 << 20) / ((int64_t)std::max(ns, 1) * 4));
+  int rc = b.begin(ctx, ncdp, cdp_off, ns, dt_us, geom, samples, store, nbr_off, nbr, ctx->stream, rows);
+  if (rc) return rc;
+  int ready = 0;
+  if ((rc = b.batch(0, ncdp, nullptr, &ready))) return rc;
+  return b.finish(out);
 }
 
 extern "C" int sb_gather_info(const sb_gather* g, int* ncdp, int* ntr, int* ns, int* store, size_t* bytes) {
@@ -910,6 +978,11 @@ extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
     ctx->persistent = value ? 1 : 0;
     return SB_OK;
   }
+  if (option == SB_OPT_STREAM_ROWS) {
+    if (value < 0) return fail(SB_ERR_INVALID, "SB_OPT_STREAM_ROWS must be >= 0");
+    ctx->stream_rows = value;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -917,6 +990,10 @@ extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
   if (!ctx || !value) return fail(SB_ERR_INVALID, "null argument");
   if (option == SB_OPT_PERSISTENT) {
     *value = ctx->persistent;
+    return SB_OK;
+  }
+  if (option == SB_OPT_STREAM_ROWS) {
+    *value = ctx->stream_rows;
     return SB_OK;
   }
   return fail(SB_ERR_INVALID, "unknown option");
@@ -994,4 +1071,85 @@ extern "C" int sb_graph_launch(sb_graph* gr) {
   ctx->timing_pending = true;
   ctx->timing_graph = true;
   return SB_OK;
+}
+
+// ------------------------------------------------------ streamed search
+extern "C" int sb_search_streamed(sb_ctx* ctx, int kind, const void* params, int ncdp, const int32_t* cdp_off, int ns,
+                                  int dt_us, const sb_trace_geom* geom, const float* samples, int store,
+                                  const int32_t* nbr_off, const int32_t* nbr, sb_host_result* out, sb_gather** keep) {
+  if (!ctx || !params || !out || !cdp_off || ncdp <= 0 || ns <= 0)
+    return fail(SB_ERR_INVALID, "sb_search_streamed: bad arguments");
+  if (keep) *keep = nullptr;
+  if (kind != SB_METHOD_CMP && kind != SB_METHOD_CRS && kind != SB_GRAPH_CRS_DE)
+    return fail(SB_ERR_INVALID, "sb_search_streamed: bad kind");
+  if (kind != SB_METHOD_CMP && (!nbr_off || !nbr))
+    return fail(SB_ERR_INVALID, "sb_search_streamed: CRS needs neighbour lists");
+  HIPR(hipSetDevice(ctx->device));
+  hipStream_t cs = nullptr;
+  HIPR(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  std::vector<hipEvent_t> evs;
+  sb_result* r = nullptr;
+  int rc = SB_OK;
+  {
+    GatherBuild b;
+    // Batches of about 32 MiB of FP32 samples: small enough that the first
+    // search starts early, large enough to keep launches few.
+    const int64_t rows = ctx->stream_rows > 0 ? ctx->stream_rows : std::max<int64_t>(1, (int64_t)(32 << 20) / ((int64_t)ns * 4));
+    rc = b.begin(ctx, ncdp, cdp_off, ns, dt_us, geom, samples, store, kind == SB_METHOD_CMP ? nullptr : nbr_off,
+                 kind == SB_METHOD_CMP ? nullptr : nbr, cs, rows);
+    if (!rc) rc = sb_result_create(ctx, ncdp, ns, &r);
+    int cb = 0, searched = 0;
+    while (!rc && cb < ncdp) {
+      int ce = cb + 1;
+      while (ce < ncdp && (int64_t)cdp_off[ce + 1] - cdp_off[cb] <= rows) ce++;
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        rc = fail(SB_ERR_HIP, "event");
+        break;
+      }
+      evs.push_back(ev);
+      int ready = 0;
+      if ((rc = b.batch(cb, ce, ev, &ready))) break;
+      if (ready > searched) {
+        // The search of the CDPs that just became complete waits for their
+        // batch only; the next batch's copy proceeds meanwhile.
+        if (hipStreamWaitEvent(ctx->stream, ev, 0) != hipSuccess) {
+          rc = fail(SB_ERR_HIP, "stream wait");
+          break;
+        }
+        sb_result view = *r;  // rows [searched, ready)
+        const size_t shift = (size_t)searched * ns;
+        view.ncdp = ready - searched;
+        view.index += shift;
+        view.a += shift;
+        view.b += shift;
+        view.c += shift;
+        view.coher += shift;
+        view.stack += shift;
+        view.keys += shift;
+        const sb_gather* g = b.G.get();
+        if (kind == SB_METHOD_CMP)
+          rc = sb_cmp_run_range(ctx, g, (const sb_cmp_params*)params, searched, ready, &view);
+        else if (kind == SB_METHOD_CRS)
+          rc = sb_crs_run_range(ctx, g, (const sb_crs_params*)params, searched, ready, &view);
+        else
+          rc = sb_crs_de_run_range(ctx, g, (const sb_de_params*)params, searched, ready, &view);
+        searched = ready;
+      }
+      cb = ce;
+    }
+    if (!rc && searched != ncdp) rc = fail(SB_ERR_STATE, "streamed search left CDPs unsearched");
+    if (!rc) rc = sb_result_download(ctx, r, out);
+    sb_gather* g = nullptr;
+    if (!rc) rc = b.finish(&g);
+    if (!rc && keep)
+      *keep = g;
+    else if (g)
+      sb_gather_destroy(g);
+  }
+  if (r) sb_result_destroy(r);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t ev : evs) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(cs);
+  return rc;
 }

@@ -38,6 +38,7 @@ struct sb_ctx {
   unsigned long long gather_serial = 0;
   void* queue_buf = nullptr;   // persistent-grid item counters
   int persistent = 0;          // SB_OPT_PERSISTENT
+  int stream_rows = 0;         // SB_OPT_STREAM_ROWS (0: automatic)
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured
 };
