@@ -56,7 +56,9 @@ enum { SB_STORE_F32 = 0, SB_STORE_F16 = 1, SB_STORE_BF16 = 2 };
  *  SB_MODE_FAST : FP32 search (contracted FMAs, hardware sqrt, window sums
  *                 reassociated); the winning parameter of every (CDP, t0) is
  *                 then re-evaluated in the reference's precision and
- *                 operation order (CMP float, CRS double).
+ *                 operation order (CMP float, CRS double).  Windows of more
+ *                 than 7 samples (and CMP grids with c < 0) run as
+ *                 SB_MODE_EXACT.
  *  SB_MODE_EXACT: the whole search in the reference's precision and
  *                 operation order (CMP float, CRS double), no contraction,
  *                 correctly rounded sqrt/div.  Bit-identical to the

@@ -667,8 +667,10 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   bool lds = false;
   FastVariant v{};
   a.status = r->status;
+  // Windows the fast kernel is not compiled for (w > 7) run in reference
+  // arithmetic: slower, same parity bar.
+  if (mode == SB_MODE_FAST && !fast_supported(a.w)) mode = SB_MODE_EXACT;
   if (mode == SB_MODE_FAST) {
-    if (!fast_supported(a.w)) return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
     variant = ctx->fast_variant;
     v = kFastVariants[variant];
     if (v.lds_kb > 0 && a.method == SB_METHOD_CMP && lds_fits(g, a, v, cb, ce)) {
@@ -871,8 +873,9 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
   a.tau = (int)tt > 0 ? (int)tt : 0;
   a.w = 2 * a.tau + 1;
   if (a.w > SB_MAX_W) return fail(SB_ERR_UNSUPPORTED, "window 2*tau+1 exceeds SB_MAX_W");
-  if (p->mode == SB_MODE_FAST && !fast_supported(a.w))
-    return fail(SB_ERR_UNSUPPORTED, "window size not supported by the fast kernel");
+  // Windows the fast evaluator is not compiled for (w > 7) use the
+  // reference-order evaluator.
+  const int de_mode = p->mode == SB_MODE_FAST && !fast_supported(a.w) ? SB_MODE_EXACT : p->mode;
   float dt = (float)g->dt_us;
   dt = dt / 1000000.0f;
   a.dt_f = dt;
@@ -909,7 +912,7 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
   a.nwork = nblocks;
   HIPR(record(ctx, 0));
   HIPR(record(ctx, 1));
-  hipError_t e = launch_de(a, d, g->store, p->mode == SB_MODE_EXACT, nblocks, ctx->stream);
+  hipError_t e = launch_de(a, d, g->store, de_mode == SB_MODE_EXACT, nblocks, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "CRS-DE launch");
   HIPR(record(ctx, 2));
   HIPR(record(ctx, 3));
