@@ -88,6 +88,14 @@ typedef struct sb_su_gather sb_su_gather; /* a binned SU file in host memory */
  * and linearizes samples to [ntr][ns] (su_gather::linearize,
  * CMP/OpenMP/src/su_gather.cpp:73-106). */
 int sb_su_load(const char* path, int method, int aph, int apm, sb_su_gather** out);
+/* The same from a SEG-Y file (3200-byte textual + 400-byte binary header,
+ * optional rev 1 extended textual headers, big-endian trace headers and
+ * samples in format 1 = IBM float, 2 = int32, 3 = int16, 5 = IEEE float or
+ * 8 = int8; a trace's ns/dt of 0 take the binary header's).  Headers are
+ * converted to the SU (native-endian) layout, so sections written with
+ * sb_su_write_section are SU files as the reference writes them.  An
+ * extension: the reference reads SU only. */
+int sb_segy_load(const char* path, int method, int aph, int apm, sb_su_gather** out);
 void sb_su_free(sb_su_gather* g);
 
 /* Sizes of a loaded gather.  nnbr = total neighbour-list length (CRS). */

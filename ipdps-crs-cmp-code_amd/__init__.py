@@ -86,7 +86,7 @@ _lib = None
 
 # Every symbol include/semblance.h declares (checked by the tests).
 EXPORTS = [
-    "sb_strerror", "sb_last_error", "sb_version", "sb_su_load", "sb_su_free", "sb_su_info", "sb_su_cdp_offsets",
+    "sb_strerror", "sb_last_error", "sb_version", "sb_su_load", "sb_segy_load", "sb_su_free", "sb_su_info", "sb_su_cdp_offsets",
     "sb_su_samples", "sb_su_geom", "sb_su_nbr_offsets", "sb_su_nbr", "sb_su_write_section", "sb_su_write_traces",
     "sb_crs_neighbors", "sb_keep_trace", "sb_ctx_create", "sb_ctx_destroy", "sb_ctx_memory", "sb_ctx_sync",
     "sb_gather_create", "sb_gather_destroy", "sb_gather_info", "sb_result_create", "sb_result_destroy",
@@ -112,6 +112,7 @@ def lib():
         L.sb_last_error.restype = C.c_char_p
         L.sb_version.restype = C.c_char_p
         L.sb_su_load.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(P)]
+        L.sb_segy_load.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(P)]
         L.sb_su_free.argtypes = [P]
         L.sb_su_free.restype = None
         L.sb_su_info.argtypes = [P] + [C.POINTER(C.c_int)] * 5
@@ -229,12 +230,13 @@ def write_traces(path, geom, samples, dt_us):
 
 
 class SuGather:
-    """A binned SU file (sb_su_load)."""
+    """A binned SU file (sb_su_load), or a SEG-Y file with segy=True
+    (sb_segy_load)."""
 
-    def __init__(self, path, method, aph, apm=0):
+    def __init__(self, path, method, aph, apm=0, segy=False):
         L = lib()
         h = C.c_void_p()
-        check(L.sb_su_load(path.encode(), method, aph, apm, C.byref(h)))
+        check((L.sb_segy_load if segy else L.sb_su_load)(path.encode(), method, aph, apm, C.byref(h)))
         self.h = h
         v = [C.c_int() for _ in range(5)]
         check(L.sb_su_info(h, *[C.byref(x) for x in v]))

@@ -13,6 +13,7 @@
 //   SB_DEVICE = HIP device index (default 0)
 #pragma once
 
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -135,8 +136,16 @@ struct Session {
     if (ctx) sb_ctx_destroy(ctx);
     if (su) sb_su_free(su);
   }
+  // SU input as the reference; files named *.sgy / *.segy are read as SEG-Y.
   void load(const std::string& path, int method, int aph, int apm) {
-    check(sb_su_load(path.c_str(), method, aph, apm, &su), "reading input");
+    std::string low = path;
+    for (auto& ch : low) ch = (char)std::tolower((unsigned char)ch);
+    auto ends = [&](const char* x) {
+      const size_t n = std::strlen(x);
+      return low.size() >= n && low.compare(low.size() - n, n, x) == 0;
+    };
+    const bool segy = ends(".sgy") || ends(".segy");
+    check((segy ? sb_segy_load : sb_su_load)(path.c_str(), method, aph, apm, &su), "reading input");
     check(sb_su_info(su, &ncdp, &ntr, &ns, &dt_us, &nnbr), "input");
   }
   void open(int device) { check(sb_ctx_create(device, 0, &ctx), "device"); }

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 
@@ -118,29 +119,30 @@ extern "C" int sb_crs_neighbors(int ncdp, const sb_trace_geom* first_geom, int a
   return SB_OK;
 }
 
-extern "C" int sb_su_load(const char* path, int method, int aph, int apm, sb_su_gather** out) {
-  if (!path || !out || (method != SB_METHOD_CMP && method != SB_METHOD_CRS))
-    return fail(SB_ERR_INVALID, "sb_su_load: bad arguments");
-  *out = nullptr;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return fail(SB_ERR_IO, std::string("cannot open ") + path);
-  std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+namespace {
 
+// Source of traces in file order: header in the SU (native-endian SEG-Y
+// trace header) layout and the samples as FP32.  Returns 1 for a trace, 0 at
+// the end of the data, or a negative status after fail().
+using NextTrace = std::function<int(SuHeader&, std::vector<float>&)>;
+
+// The reference's binning (CMP/OpenMP/src/su_gather.cpp:33-106,
+// CRS/OpenMP/src/su_gather.cpp:33-90): aph filter, std::map key order, CMP
+// fold-descending std::sort, CRS apm neighbour lists; common to SU and SEG-Y.
+int build_su_gather(const NextTrace& next, int method, int aph, int apm, sb_su_gather** out) {
   struct Cdp {
     std::vector<SuHeader> hdr;
     std::vector<float> data;
   };
   std::map<int, Cdp> by_cdp;
-  int ns = -1, dt_first = 0;
+  int ns = -1;
   SuHeader h;
   std::vector<float> buf;
   while (true) {
-    size_t got = std::fread(h.raw, 1, kSuHeaderBytes, f);
-    if (got < (size_t)kSuHeaderBytes) break;  // end of file (a partial header ends the read, as fgettr)
-    int tns = rd<uint16_t>(h.raw, kOffNs);
-    buf.resize(tns);
-    if (std::fread(buf.data(), sizeof(float), tns, f) != (size_t)tns)
-      return fail(SB_ERR_IO, "Bad input data");
+    const int got = next(h, buf);
+    if (got < 0) return -got;
+    if (got == 0) break;
+    const int tns = (int)buf.size();
     sb_trace_geom g = geom_of(h.raw);
     if (!sb_keep_trace(&g, method, aph)) continue;
     if (ns < 0) ns = tns;
@@ -150,7 +152,6 @@ extern "C" int sb_su_load(const char* path, int method, int aph, int apm, sb_su_
     c.data.insert(c.data.end(), buf.begin(), buf.end());
   }
   if (by_cdp.empty()) return fail(SB_ERR_FORMAT, "no trace passed the aperture test");
-  (void)dt_first;
 
   // CDP order.  CMP: std::sort with the reference comparator (fold descending)
   // over the std::map order; the permutation depends only on comparator
@@ -205,6 +206,140 @@ extern "C" int sb_su_load(const char* path, int method, int aph, int apm, sb_su_
   *out = G.release();
   return SB_OK;
 }
+
+}  // namespace
+
+extern "C" int sb_su_load(const char* path, int method, int aph, int apm, sb_su_gather** out) {
+  if (!path || !out || (method != SB_METHOD_CMP && method != SB_METHOD_CRS))
+    return fail(SB_ERR_INVALID, "sb_su_load: bad arguments");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(SB_ERR_IO, std::string("cannot open ") + path);
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+  // su_trace fgettr (CMP/OpenMP/src/su_trace.cpp:37-54): a partial header
+  // ends the read; a short sample block is "Bad input data".
+  auto next = [&](SuHeader& h, std::vector<float>& buf) -> int {
+    if (std::fread(h.raw, 1, kSuHeaderBytes, f) < (size_t)kSuHeaderBytes) return 0;
+    const int tns = rd<uint16_t>(h.raw, kOffNs);
+    buf.resize(tns);
+    if (std::fread(buf.data(), sizeof(float), tns, f) != (size_t)tns) return -fail(SB_ERR_IO, "Bad input data");
+    return 1;
+  };
+  return build_su_gather(next, method, aph, apm, out);
+}
+
+namespace {
+
+// SEG-Y rev 1 trace header field widths, in order from byte 0 (the same
+// fields, at the same offsets, as the SU header).
+struct Field {
+  int off, size;
+};
+std::vector<Field> segy_trace_fields() {
+  std::vector<Field> f;
+  auto run = [&](int off, int n, int size) {
+    for (int i = 0; i < n; i++) f.push_back({off + i * size, size});
+  };
+  run(0, 7, 4);     // tracl .. cdpt
+  run(28, 4, 2);    // trid, nvs, nhs, duse
+  run(36, 8, 4);    // offset .. gwdep
+  run(68, 2, 2);    // scalel, scalco
+  run(72, 4, 4);    // sx, sy, gx, gy
+  run(88, 46, 2);   // counit .. (bytes 89-180)
+  run(180, 5, 4);   // cdp x/y, inline, crossline, shotpoint
+  run(200, 2, 2);   // shotpoint scalar, trace value unit
+  run(204, 1, 4);   // transduction constant mantissa
+  run(208, 7, 2);   // exponent, units, device id, time scalar, source type, energy direction
+  run(222, 1, 2);   // energy direction (last)
+  run(224, 1, 4);   // source measurement mantissa
+  run(228, 2, 2);   // exponent, unit
+  run(232, 2, 4);   // unassigned
+  return f;
+}
+
+uint32_t be32(const uint8_t* p) { return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]; }
+uint16_t be16(const uint8_t* p) { return (uint16_t)(((uint16_t)p[0] << 8) | p[1]); }
+
+// IBM System/360 single precision: sign, 7-bit excess-64 base-16 exponent,
+// 24-bit fraction 0.F; value = (-1)^s * F * 2^-24 * 16^(e-64).  Exact in
+// double, then rounded once to FP32.
+float ibm_to_float(uint32_t u) {
+  const uint32_t frac = u & 0x00ffffffu;
+  if (frac == 0) return (u >> 31) ? -0.0f : 0.0f;
+  const int e = (int)((u >> 24) & 0x7f);
+  const double v = std::ldexp((double)frac, 4 * (e - 64) - 24);
+  return (float)((u >> 31) ? -v : v);
+}
+
+}  // namespace
+
+extern "C" int sb_segy_load(const char* path, int method, int aph, int apm, sb_su_gather** out) {
+  if (!path || !out || (method != SB_METHOD_CMP && method != SB_METHOD_CRS))
+    return fail(SB_ERR_INVALID, "sb_segy_load: bad arguments");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(SB_ERR_IO, std::string("cannot open ") + path);
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+  // 3200-byte textual header, 400-byte binary header (big-endian).
+  uint8_t text[3200], bin[400];
+  if (std::fread(text, 1, 3200, f) != 3200 || std::fread(bin, 1, 400, f) != 400)
+    return fail(SB_ERR_FORMAT, "SEG-Y: file shorter than its 3600-byte headers");
+  const int bin_dt = be16(bin + 16), bin_ns = be16(bin + 20), format = be16(bin + 24);
+  const int next_text = (int16_t)be16(bin + 304);  // rev 1 extended textual headers (-1: variable)
+  if (next_text < 0) return fail(SB_ERR_UNSUPPORTED, "SEG-Y: variable number of extended textual headers");
+  if (next_text > 0 && std::fseek(f, 3200L * next_text, SEEK_CUR) != 0)
+    return fail(SB_ERR_FORMAT, "SEG-Y: extended textual headers missing");
+  int bytes = 0;
+  switch (format) {
+    case 1: case 2: case 5: bytes = 4; break;  // IBM float, int32, IEEE float
+    case 3: bytes = 2; break;                  // int16
+    case 8: bytes = 1; break;                  // int8
+    default: return fail(SB_ERR_UNSUPPORTED, "SEG-Y: sample format code " + std::to_string(format));
+  }
+  const std::vector<Field> fields = segy_trace_fields();
+  std::vector<uint8_t> raw;
+  uint8_t th[kSuHeaderBytes];
+  auto next = [&](SuHeader& h, std::vector<float>& buf) -> int {
+    const size_t got = std::fread(th, 1, kSuHeaderBytes, f);
+    if (got == 0) return 0;
+    if (got < (size_t)kSuHeaderBytes) return -fail(SB_ERR_IO, "SEG-Y: truncated trace header");
+    for (const Field& fd : fields) {  // big-endian fields -> native layout
+      if (fd.size == 4) {
+        const uint32_t v = be32(th + fd.off);
+        std::memcpy(h.raw + fd.off, &v, 4);
+      } else {
+        const uint16_t v = be16(th + fd.off);
+        std::memcpy(h.raw + fd.off, &v, 2);
+      }
+    }
+    int tns = rd<uint16_t>(h.raw, kOffNs);
+    if (tns == 0) {  // per-trace ns and dt default to the binary header's
+      tns = bin_ns;
+      wr<uint16_t>(h.raw, kOffNs, (uint16_t)tns);
+    }
+    if (rd<uint16_t>(h.raw, kOffDt) == 0) wr<uint16_t>(h.raw, kOffDt, (uint16_t)bin_dt);
+    raw.resize((size_t)tns * bytes);
+    if (std::fread(raw.data(), 1, raw.size(), f) != raw.size()) return -fail(SB_ERR_IO, "Bad input data");
+    buf.resize(tns);
+    for (int i = 0; i < tns; i++) {
+      const uint8_t* p = raw.data() + (size_t)i * bytes;
+      switch (format) {
+        case 1: buf[i] = ibm_to_float(be32(p)); break;
+        case 2: buf[i] = (float)(int32_t)be32(p); break;
+        case 3: buf[i] = (float)(int16_t)be16(p); break;
+        case 5: {
+          const uint32_t u = be32(p);
+          std::memcpy(&buf[i], &u, 4);
+          break;
+        }
+        default: buf[i] = (float)(int8_t)p[0]; break;
+      }
+    }
+    return 1;
+  };
+  return build_su_gather(next, method, aph, apm, out);
+}
+
 
 extern "C" void sb_su_free(sb_su_gather* g) { delete g; }
 

@@ -79,3 +79,18 @@ def test_cli_crs_de_runs(manifest, tmp_path):
     assert p.returncode == 0, p.stderr
     for f in ("crs.a.su", "crs.b.su", "crs.c.su", "crs.coher.su", "crs.stack.su"):
         assert os.path.getsize(str(tmp_path / f)) == os.path.getsize(os.path.join(GOLD, "crs_w3", f))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,prog", [("cmp_w3", "cmp"), ("crs_w3", "crs")])
+def test_cli_reads_segy_input(manifest, tmp_path, case, prog):
+    """A SEG-Y file (*.sgy, big-endian, IEEE samples) carrying the fixture's
+    traces gives the reference's output files byte for byte."""
+    from segy_util import read_su, write_segy
+    print("case: cli segy", case, flush=True)
+    write_segy(str(tmp_path / "input.sgy"), read_su(os.path.join(GOLD, case, "input.su")), 5, 0)
+    c = manifest[case]
+    p = run(os.path.join(LIB, prog), c["args"] + ["-i", "input.sgy", "-v", "2"], str(tmp_path), {"SB_MODE": "exact"})
+    assert p.returncode == 0, p.stderr
+    for f in c["outputs"]:
+        assert filecmp.cmp(str(tmp_path / f), os.path.join(GOLD, case, f), shallow=False), (case, f)
