@@ -12,6 +12,8 @@
 // lanes holds one (CDP, t0), so a wave carries 64/P2 consecutive t0.  Each
 // lane walks the CDP's supergather serially for its own candidate; the
 // mutation reads r1/r2/r3 from sibling lanes with ds_bpermute.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "launch.hpp"
 
@@ -136,10 +138,14 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   int cnt = 0;
 #pragma unroll
   for (int j = 0; j < W; j++) num[j] = 0.f;
-  SlotF nx = a.slots[k0 < k1 ? k0 : 0];
-  for (int k = k0; k < k1; k++) {
-    const SlotF s = nx;
-    nx = a.slots[k + 1 < k1 ? k + 1 : k];
+  // Software-pipelined by one trace: the window of trace k+1 is loaded
+  // while trace k accumulates (one load per candidate and trace, so without
+  // this every wave waits out each load's latency).  Same order of sums.
+  // FP16 windows stay halves and feed mixed-precision FMAs (v_fma_mix_f32);
+  // BF16 and FP32 windows are FP32.
+  constexpr bool HALF = STORE == SB_STORE_F16;
+  typedef typename std::conditional<HALF, WinH<W + 1>, Win<W + 1>>::type WinT;
+  auto issue = [&](const SlotF& s, WinT& sv, float& x, bool& ok) {
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
     const float y = fmaf(pa, s.m, T0);
@@ -148,11 +154,16 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
     // fraction finite; the window test rejects it unless tau = 0.
     const float ts = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.f));
     const int e = (int)ts - a.tau;
-    bool ok = (unsigned)e < nwin;
+    ok = (unsigned)e < nwin;
     if constexpr (W == 1) ok = ok && tt >= 0.f;
-    const float x = __builtin_amdgcn_fractf(ts);
+    x = __builtin_amdgcn_fractf(ts);
     const unsigned off = ok ? (STORE == SB_STORE_F32 ? (unsigned)e * 4u : ((unsigned)e & ~1u) * 2u) : kOob;
-    const Win<W + 1> sv = load_win<W + 1, STORE>(rs, off, ((unsigned)e & 1u) << 4);
+    if constexpr (HALF)
+      sv = load_win_h<W + 1>(rs, off, ((unsigned)e & 1u) << 4);
+    else
+      sv = load_win<W + 1, STORE>(rs, off, ((unsigned)e & 1u) << 4);
+  };
+  auto consume = [&](const SlotF& s, const WinT& sv, float x, bool ok) {
     cnt += ok ? 1 : 0;
     if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll
@@ -161,6 +172,17 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
         num[j] += v;
         den = fmaf(v, v, den);
       }
+    } else if constexpr (HALF) {
+      // v = s_j + x s_{j+1} - x s_j in two mixed-precision FMAs
+      float q = 0.f;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const float hj = (float)sv.h[j], hj1 = (float)sv.h[j + 1];
+        const float v = fmaf(-x, hj, fmaf(x, hj1, hj));
+        num[j] = fmaf(s.scale, v, num[j]);
+        q = fmaf(v, v, q);
+      }
+      den = fmaf(s.scale2, q, den);
     } else {
       float q = 0.f;
 #pragma unroll
@@ -170,6 +192,22 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
         q = fmaf(v, v, q);
       }
       den = fmaf(s.scale2, q, den);
+    }
+  };
+  if (k0 < k1) {
+    WinT wa, wb;
+    float xa, xb;
+    bool oa, ob;
+    SlotF sa = a.slots[k0];
+    issue(sa, wa, xa, oa);
+    for (int k = k0; k < k1; k += 2) {
+      const SlotF sb = a.slots[k + 1 < k1 ? k + 1 : k];
+      issue(sb, wb, xb, ob);  // a repeat of trace k at the end: loaded, never summed
+      consume(sa, wa, xa, oa);
+      if (k + 1 >= k1) break;
+      sa = a.slots[k + 2 < k1 ? k + 2 : k + 1];
+      issue(sa, wa, xa, oa);
+      consume(sb, wb, xb, ob);
     }
   }
   float acsq = 0.f, lin = 0.f;

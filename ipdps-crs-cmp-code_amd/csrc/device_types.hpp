@@ -39,6 +39,10 @@ struct WorkItem {
   int32_t p0, p1;  // parameter range [p0, p1)
 };
 
+// Bytes of device memory before row 0 of the samples (at least SB_MAX_W / 2
+// FP32 samples, a multiple of 256 so rows stay 256-byte aligned).
+constexpr int kSamplesLead = 256;
+
 // Everything a search kernel reads.  Device pointers.
 struct SearchArgs {
   const void* samples;     // [ntr][stride] in the store type
@@ -61,6 +65,12 @@ struct SearchArgs {
   float cf0, cfinc;                              // CMP: float c0*1e6, inc
   float dt_f, idt_f;                             // CMP float dt, idt
   double dt_d, idt_d;                            // CRS double dt, idt
+  // CMP fast search: a window is valid iff t^2 = T0^2 + c h < hi2 (and its
+  // start lies in [0, ns - w)), hi2 = (ns - tau - 1)^2 lowered by 2^-20 so
+  // that the hardware square root of any t^2 below it starts a window
+  // inside the trace.  A bound on t^2 is monotone in h, which the prefix
+  // path of the search relies on.
+  float hi2;
   // outputs
   unsigned long long* keys;  // [nout][ns] packed best (sem bits << 32 | ~p)
   int32_t* out_index;

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
@@ -66,22 +67,46 @@ __global__ void k_prep_traces(const TraceGeom* __restrict__ g, int ntr, float* _
   }
 }
 
-// CMP slot table: slot k of CDP i is row cdp_off[i] + k, H = h * idt^2.
-// Rows [row0, row0 + n).
-__global__ void k_cmp_slots(int row0, int n, const double* __restrict__ h0, const float* __restrict__ scale,
-                            double idt, SlotF* __restrict__ slots) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int r = row0 + i;
-  SlotF s;
-  s.row = r;
-  s.m = 0.f;
-  s.m2 = 0.f;
-  s.h = (float)(h0[r] * idt * idt);
-  s.scale = scale ? scale[r] : 1.f;
-  s.scale2 = s.scale * s.scale;
-  s.pad0 = s.pad1 = 0;
-  slots[r] = s;
+// CMP slot table of CDPs [cdp0, cdp0 + gridDim.x), one block per CDP: the
+// CDP's traces ordered by increasing H = h * idt^2 (FP32, ties by row), so
+// that the valid traces of any (t0, c >= 0) form a prefix of the list (the
+// fast search's prefix path).  The fast search sums in this order; the
+// reference-order evaluations read rows directly.
+__global__ __launch_bounds__(256) void k_cmp_slots(int cdp0, const int32_t* __restrict__ cdp_off,
+                                                   const double* __restrict__ h0, const float* __restrict__ scale,
+                                                   double idt, SlotF* __restrict__ slots) {
+  __shared__ float s_h[1024];
+  const int i = cdp0 + blockIdx.x;
+  const int r0 = cdp_off[i], r1 = cdp_off[i + 1];
+  const int n = r1 - r0;
+  for (int rb = 0; rb < n; rb += blockDim.x) {
+    const int r = r0 + rb + (int)threadIdx.x;
+    const bool mine = r < r1;
+    const float h = mine ? (float)(h0[r] * idt * idt) : 0.f;
+    int rank = 0;
+    for (int qb = 0; qb < n; qb += 1024) {
+      const int m = min(1024, n - qb);
+      __syncthreads();
+      for (int q = threadIdx.x; q < m; q += blockDim.x) s_h[q] = (float)(h0[r0 + qb + q] * idt * idt);
+      __syncthreads();
+      if (mine)
+        for (int q = 0; q < m; q++) {
+          const float hq = s_h[q];
+          rank += (hq < h || (hq == h && qb + q < r - r0)) ? 1 : 0;
+        }
+    }
+    if (mine) {
+      SlotF s;
+      s.row = r;
+      s.m = 0.f;
+      s.m2 = 0.f;
+      s.h = h;
+      s.scale = scale ? scale[r] : 1.f;
+      s.scale2 = s.scale * s.scale;
+      s.pad0 = s.pad1 = 0;
+      slots[r0 + rank] = s;
+    }
+  }
 }
 
 // CRS supergather slot tables, one block per CDP (CDPs cdp0 + blockIdx.x):
@@ -467,6 +492,16 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
   const int nrec = (int)(a.stride * ESZ);
   const char* __restrict__ base = (const char*)a.samples;
 
+  // CMP tiles whose every t0 exceeds tau take the prefix path.  The CMP slot
+  // table lists each CDP's traces by increasing h and t^2 = T0^2 + c h
+  // (c >= 0) is monotone in h, so the valid traces of a (t0, c) are a prefix
+  // of the list, and a result that tolerates one invalid trace can only lose
+  // the last one.  The traces before it therefore run with no validity work
+  // at all (a lane whose prefix is shorter is rejected afterwards by the
+  // second-to-last trace's test, whatever its windows read), and only the
+  // last trace is gated.  Sums, order and validity equal the general path's.
+  const bool prefix = !CRS && wi.tile * 64 * R > a.tau;
+
   for (int p = pw0; p < pw1; p += CC) {
     float pa[CC], pb[CC], pc[CC];
     bool act[CC];
@@ -498,12 +533,18 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
         for (int j = 0; j < W; j++) num[r][ci][j] = 0.f;
       }
 
-    SlotF nx = sl[0];
-    for (int k = 0; k < S; k++) {
-      const SlotF s = nx;
-      nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
-      const __amdgpu_buffer_rsrc_t rs =
-          __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
+    // One trace of the CDP for every (row, parameter) of the lane.
+    // GATE 0: full validity test and count (general path); 1: none (prefix
+    // path, all but the last trace); 2: the t^2 bound only (prefix path,
+    // last trace).
+    auto trace = [&](const SlotF& s, auto gate_tag) {
+      constexpr int GATE = decltype(gate_tag)::value;
+      // FP32 rows are addressed from tau samples before their start (the
+      // allocation has kSamplesLead bytes in front of row 0), so a window's
+      // byte offset is 4 * floor(t).
+      const int lead = STORE == SB_STORE_F32 ? 4 * a.tau : 0;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(base + (int64_t)s.row * a.stride * ESZ - lead), (short)0, nrec + lead, kRsrcWord3);
       // Every rounding step below is explicit (fmaf / __fmul_rn), so an
       // evaluation's bits do not depend on the launch shape or batch.
       float U[CC];
@@ -516,7 +557,6 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
       }
       unsigned off[R][CC], shf[R][CC];
       float xx[R][CC];
-      bool okk[R][CC];
 #pragma unroll
       for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -529,7 +569,6 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
             tt = fmaf(pc[ci], s.h, T0sq[r]);
           }
           float ts;
-          bool ok;
           if constexpr (CRS) {
             // t^2 < 0 is invalid (the reference sets t = -1): clamp so the
             // fraction stays finite, then the window test rejects it unless
@@ -540,13 +579,20 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
           }
           // Window start e = it - tau must lie in [0, ns - w): the
           // reference's it - tau >= 0 && it + tau + 1 < ns.
-          const int e = (int)ts - a.tau;
-          ok = (unsigned)e < nwin;
-          if constexpr (CRS && W == 1) ok = ok && tt >= 0.f;
-          okk[r][ci] = ok;
+          const int it = (int)ts;
+          const int e = it - a.tau;
+          bool ok = true;
+          if constexpr (GATE == 0) {
+            ok = (unsigned)e < nwin;
+            if constexpr (CRS && W == 1) ok = ok && tt >= 0.f;
+            if constexpr (!CRS) ok = ok && tt < a.hi2;
+          } else if constexpr (GATE == 2) {
+            ok = tt < a.hi2;
+          }
+          if constexpr (GATE != 1) cnt[r][ci] += ok ? 1 : 0;
           xx[r][ci] = __builtin_amdgcn_fractf(ts);
           if constexpr (STORE == SB_STORE_F32) {
-            off[r][ci] = ok ? (unsigned)e * 4u : kOob;
+            off[r][ci] = ok ? (unsigned)it * 4u : kOob;
           } else {
             off[r][ci] = ok ? ((unsigned)e & ~1u) * 2u : kOob;
             shf[r][ci] = ((unsigned)e & 1u) << 4;
@@ -569,7 +615,6 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
       for (int r = 0; r < R; r++) {
 #pragma unroll
         for (int ci = 0; ci < CC; ci++) {
-          cnt[r][ci] += okk[r][ci] ? 1 : 0;
           const float x = xx[r][ci];
           if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll
@@ -602,6 +647,38 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
           }
         }
       }
+    };
+
+    // Whether each lane's traces before the last are all valid (prefix
+    // path): the second-to-last trace's t^2 bound decides it.
+    bool head_ok[R][CC];
+    if (!prefix) {
+      SlotF nx = sl[0];
+      for (int k = 0; k < S; k++) {
+        const SlotF s = nx;
+        nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
+        trace(s, std::integral_constant<int, 0>{});
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) head_ok[r][ci] = (S - cnt[r][ci]) < 2;
+    } else {
+      SlotF nx = sl[0];
+      for (int k = 0; k + 1 < S; k++) {
+        const SlotF s = nx;
+        nx = sl[k + 1];
+        trace(s, std::integral_constant<int, 1>{});
+      }
+      trace(nx, std::integral_constant<int, 2>{});
+      const float h2 = S >= 2 ? sl[S - 2].h : 0.f;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) {
+          head_ok[r][ci] = S < 2 || fmaf(pc[ci], h2, T0sq[r]) < a.hi2;
+          cnt[r][ci] += S - 1;
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -611,7 +688,7 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
 #pragma unroll
         for (int j = 0; j < W; j++) acsq = fmaf(num[r][ci][j], num[r][ci][j], acsq);
         const float m = (float)cnt[r][ci];
-        const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && (S - cnt[r][ci]) < 2;
+        const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && head_ok[r][ci];
         const float sem = __fmul_rn(acsq, __builtin_amdgcn_rcpf(__fmul_rn(den[r][ci], m)));
         if (valid && sem > 0.f) {
           const unsigned long long key =
@@ -678,10 +755,10 @@ hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, doubl
   return SB_CHECK_LAUNCH();
 }
 
-hipError_t launch_cmp_slots(int row0, int n, const double* h0, const float* scale, double idt, SlotF* slots,
-                            hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cmp_slots, dim3((n + 255) / 256), dim3(256), 0, st, row0, n, h0, scale, idt, slots);
+hipError_t launch_cmp_slots(int cdp0, int ncdp, const int32_t* cdp_off, const double* h0, const float* scale,
+                            double idt, SlotF* slots, hipStream_t st) {
+  if (ncdp <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cmp_slots, dim3(ncdp), dim3(256), 0, st, cdp0, cdp_off, h0, scale, idt, slots);
   return SB_CHECK_LAUNCH();
 }
 

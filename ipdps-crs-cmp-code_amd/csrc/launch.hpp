@@ -26,8 +26,9 @@ bool fast_supported(int w);
 
 hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
                               hipStream_t st);
-hipError_t launch_cmp_slots(int row0, int n, const double* h0, const float* scale, double idt, SlotF* slots,
-                            hipStream_t st);
+// CMP slot tables of CDPs [cdp0, cdp0 + ncdp): each CDP's traces by increasing h.
+hipError_t launch_cmp_slots(int cdp0, int ncdp, const int32_t* cdp_off, const double* h0, const float* scale,
+                            double idt, SlotF* slots, hipStream_t st);
 hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
                             const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
                             const float* scale, double idt, SlotF* sf, SlotD* sd, SlotF* sp, hipStream_t st);

@@ -298,7 +298,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
             const float tt = fmaf(pc[ci], h, T0sq[r]);
             const float ts = __builtin_amdgcn_sqrtf(tt);  // tt >= 0: the host checks c0, c1 >= 0
             const int e = (int)ts - a.tau;
-            const bool ok = (unsigned)e < (unsigned)nwin;
+            const bool ok = (unsigned)e < (unsigned)nwin && tt < a.hi2;  // the global kernel's rule
             okk[r][ci] = ok;
             xx[r][ci] = __builtin_amdgcn_fractf(ts);
             read_win<W, STORE>(pairs, ok ? (unsigned)(sbk + e) : 0u, w[r][ci]);

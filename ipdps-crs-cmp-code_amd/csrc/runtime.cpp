@@ -196,7 +196,7 @@ extern "C" void sb_gather_destroy(sb_gather* g) {
   // Searches from other contexts may still read the gather; its blocks go
   // back to the cache for reuse, so wait for the whole device.
   (void)hipDeviceSynchronize();
-  for (void* p : {(void*)g->d_samples, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
+  for (void* p : {(void*)g->d_samples_mem, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
                   (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
                   (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p})
     ctx_free(c, p);
@@ -288,7 +288,10 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
   G->cmp_hmax = INFINITY;  // known once every slot exists (finish)
   const size_t esz = store == SB_STORE_F32 ? 4 : 2;
   int rc;
-  if ((rc = ctx_alloc(ctx, (size_t)ntr * G->stride * esz, &G->d_samples))) return rc;
+  // kSamplesLead bytes before row 0: the FP32 search addresses a row from
+  // tau samples before its start.
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * G->stride * esz + kSamplesLead, &G->d_samples_mem))) return rc;
+  G->d_samples = (char*)G->d_samples_mem + kSamplesLead;
   if (store != SB_STORE_F32 && (rc = ctx_alloc(ctx, (size_t)ntr * 4, (void**)&G->d_scale))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)(ncdp + 1) * 4, (void**)&G->d_cdp_off))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 4, (void**)&G->d_h_ref32))) return rc;
@@ -368,7 +371,7 @@ int GatherBuild::batch(int cb, int ce, hipEvent_t ev, int* ready) {
     HIPR(launch_store((const float*)st, (int)nr, ns, G->stride, store, (char*)G->d_samples + (size_t)r0 * G->stride * esz,
                       G->d_scale ? G->d_scale + r0 : nullptr, cs));
   }
-  HIPR(launch_cmp_slots((int)rb, (int)(re - rb), G->d_h0, G->d_scale, idt, G->d_cmp_slots, cs));
+  HIPR(launch_cmp_slots(cb, ce - cb, G->d_cdp_off, G->d_h0, G->d_scale, idt, G->d_cmp_slots, cs));
   int done = ce;
   if (G->has_crs) {
     // CDPs whose every neighbour lies below ce (the reach is a prefix maximum).
@@ -396,7 +399,7 @@ int GatherBuild::finish(sb_gather** out) {
   g->bytes = 0;
   {
     std::lock_guard<std::mutex> lk(ctx->mem_mu);
-    for (void* p : {(void*)g->d_samples, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
+    for (void* p : {(void*)g->d_samples_mem, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
                     (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
                     (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p}) {
       auto it = ctx->sizes.find(p);
@@ -778,6 +781,10 @@ extern "C" int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_pa
   a.npar = p->nc;
   a.dt_f = dt;
   a.idt_f = idt;
+  {
+    const int hi = g->ns - tau - 1;
+    a.hi2 = hi > 0 ? (float)((double)hi * (double)hi * (1.0 - 0x1p-20)) : 0.f;
+  }
   std::vector<int64_t> S(g->ncdp);
   for (int i = 0; i < g->ncdp; i++) S[i] = g->cdp_off[i + 1] - g->cdp_off[i];
   // The fast kernel assumes t0^2 + C h^2 >= 0; a negative C range (NaN

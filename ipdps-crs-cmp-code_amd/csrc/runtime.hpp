@@ -49,7 +49,8 @@ struct sb_gather {
   int ncdp = 0, ntr = 0, ns = 0, dt_us = 0, store = 0;
   int64_t stride = 0;
   bool has_crs = false;
-  void* d_samples = nullptr;
+  void* d_samples = nullptr;      // row 0; kSamplesLead bytes into d_samples_mem
+  void* d_samples_mem = nullptr;
   float* d_scale = nullptr;
   int32_t* d_cdp_off = nullptr;
   float* d_h_ref32 = nullptr;
