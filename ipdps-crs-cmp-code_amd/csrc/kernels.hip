@@ -652,6 +652,21 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
     // Whether each lane's traces before the last are all valid (prefix
     // path): the second-to-last trace's t^2 bound decides it.
     bool head_ok[R][CC];
+    if (prefix) {
+      const float h2 = S >= 2 ? sl[S - 2].h : 0.f;
+      bool live = false;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) {
+          head_ok[r][ci] = S < 2 || fmaf(pc[ci], h2, T0sq[r]) < a.hi2;
+          live = live || (head_ok[r][ci] && act[ci]);
+        }
+      // A lane whose second-to-last trace is invalid has at least two
+      // invalid traces and no result; when no lane of the wave has one for
+      // any of its parameters, the trace loop is skipped (same keys).
+      if (__builtin_amdgcn_ballot_w64(live) == 0ull) continue;
+    }
     if (!prefix) {
       SlotF nx = sl[0];
       for (int k = 0; k < S; k++) {
@@ -671,14 +686,10 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
         trace(s, std::integral_constant<int, 1>{});
       }
       trace(nx, std::integral_constant<int, 2>{});
-      const float h2 = S >= 2 ? sl[S - 2].h : 0.f;
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int ci = 0; ci < CC; ci++) {
-          head_ok[r][ci] = S < 2 || fmaf(pc[ci], h2, T0sq[r]) < a.hi2;
-          cnt[r][ci] += S - 1;
-        }
+        for (int ci = 0; ci < CC; ci++) cnt[r][ci] += S - 1;
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
