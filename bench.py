@@ -174,6 +174,51 @@ def cpu_baseline(sb, cdp_off, geom, samples, every=4):
         return dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port", sample=sample)
 
 
+def cli_config0(sb):
+    """The reference's own CPU-runnable case end to end through the drop-in
+    CLI: a 200-CDP line, fold 60 at half-offsets 0-1475 m (25 m steps), ns
+    1001, dt 2 ms, nc 101, tau 2 ms, written as an SU file.  Runs the `cmp`
+    program in fast and exact mode and, when built, the reference's cmp-omp2
+    on the same file and cores; reports each INFO line's rate and wall time,
+    and whether the exact-mode output files equal the reference's byte for
+    byte."""
+    import filecmp
+    cores = len(os.sched_getaffinity(0))
+    lib = os.path.join(ROOT, "ipdps-crs-cmp-code_amd", "lib", "cmp")
+    ref = os.path.join(ROOT, "oracle", "_ref", "cmp-omp2")
+    off, geom, s = sb.synth(0, 200, 60, 1001, 2000, SEED, h_max=1500.0, regular_offsets=True, h_step=25.0,
+                            nevents=8, snr=5.0)
+    args = ["-c0", "1.98e-7", "-c1", "1.77e-6", "-nc", "101", "-aph", "1500", "-tau", "0.002", "-v", "2"]
+    out = dict(workload="CMP 200 CDPs fold 60 ns 1001 nc 101 (%.4g evals), SU file -> SU sections" %
+               (float(len(geom)) * 1001 * 101))
+    names = ("cmp.c.su", "cmp.coher.su", "cmp.stack.su")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "line.su")
+        sb.write_traces(path, geom, s, 2000)
+        runs = [("fast", lib, {"SB_MODE": "fast"}), ("exact", lib, {"SB_MODE": "exact"})]
+        if os.path.exists(ref):
+            runs.append(("reference", ref, {"OMP_NUM_THREADS": str(cores), "OMP_PROC_BIND": "close"}))
+        for name, prog, env in runs:
+            d = os.path.join(td, name)
+            os.mkdir(d)
+            t = time.perf_counter()
+            p = subprocess.run([prog] + args + ["-i", path], cwd=d, capture_output=True, text=True,
+                               env=dict(os.environ, **env))
+            wall = time.perf_counter() - t
+            m = re.search(r"Semblances-Trace/s: ([0-9.eE+-]+)", p.stdout)
+            out[name] = dict(rc=p.returncode, info_evals_per_s=float(m.group(1)) * 1e9 if m else None,
+                             wall_s=round(wall, 3))
+        if "reference" in out and out["reference"]["rc"] == 0 and out["exact"]["rc"] == 0 and out["fast"]["rc"] == 0:
+            out["reference"]["cores"] = cores
+            out["exact_files_identical"] = all(
+                filecmp.cmp(os.path.join(td, "exact", n), os.path.join(td, "reference", n), shallow=False)
+                for n in names)
+            a = np.fromfile(os.path.join(td, "fast", "cmp.c.su"), np.uint8)
+            b = np.fromfile(os.path.join(td, "reference", "cmp.c.su"), np.uint8)
+            out["fast_c_section_identical"] = bool(a.shape == b.shape and np.array_equal(a, b))
+    return out
+
+
 def crs_workload(sb, cfg, seed):
     cdp_off, geom, samples = sb.synth(1, cfg["ncdp"], cfg["fold"], cfg["ns"], cfg["dt_us"], seed, h_max=cfg["h_max"],
                                       mid_half_width=cfg["mid"], nevents=cfg["nevents"], snr=cfg["snr"],
@@ -504,6 +549,8 @@ def main():
     if not args.no_side:
         del cdp_off, geom, samples
         line.update(side_configs(sb, args))
+        print("bench: config-0 line through the CLI", flush=True)
+        line["cli_config0"] = cli_config0(sb)
     if not args.no_serving:
         line["serving"] = serving_bench(sb)
         if "probe" in line["serving"]:
