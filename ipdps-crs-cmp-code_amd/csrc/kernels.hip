@@ -215,11 +215,12 @@ __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
 // ---------------------------------------------- reference-order evaluation
 // One (CDP, t0, c) of CMP exactly as CMP/OpenMP/src/main.cpp:150-203: float,
 // no contraction, correctly rounded sqrt/div, traces in gather order.
+// MAXW: the largest window the caller runs (w <= MAXW).
 // KAHAN: the window, energy and stack sums carry a compensation term
 // (Kahan summation) -- used to report 16-bit-store results, whose inputs
 // already differ from the reference's, so that FP32 accumulation adds no
 // error of its own; FP32 stores keep the reference's plain sums (and bits).
-template <bool KAHAN = false, class LD>
+template <int MAXW = SB_MAX_W, bool KAHAN = false, class LD>
 __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int cid, float& sem,
                                float& stk) {
 #pragma clang fp contract(off)
@@ -228,13 +229,13 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   tt0 = tt0 * tt0;
   const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
   const int w = a.w;
-  float num[SB_MAX_W];
+  float num[MAXW];
 #pragma unroll
-  for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.f;
+  for (int j = 0; j < MAXW; j++) num[j] = 0.f;
   float den = 0.f, lin = 0.f, m = 0.f;
-  float cnum[SB_MAX_W], cden = 0.f, clin = 0.f;  // Kahan compensations
+  float cnum[MAXW], cden = 0.f, clin = 0.f;  // Kahan compensations
 #pragma unroll
-  for (int j = 0; j < SB_MAX_W; j++) cnum[j] = 0.f;
+  for (int j = 0; j < MAXW; j++) cnum[j] = 0.f;
   auto add = [](float& sum, float& comp, float v) {
     if constexpr (KAHAN) {
       const float y = v - comp;
@@ -247,19 +248,29 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   };
   int err = 0;
   const int r0 = a.cdp_off[cdp], r1 = a.cdp_off[cdp + 1];
+  // The next row's h is loaded while this row computes, and a window's
+  // samples are all loaded before its sums (independent loads in flight;
+  // the arithmetic and its order are unchanged).
+  float h_next = a.h_ref32[r0];
   for (int row = r0; row < r1; row++) {
-    float ch = c * a.h_ref32[row];
+    const float h = h_next;
+    if (row + 1 < r1) h_next = a.h_ref32[row + 1];
+    float ch = c * h;
     float t = sqrtf(tt0 + ch) * a.idt_f;
     if (t >= lo && t < hi) {
       const int it = (int)t;
       const float x = t - (float)it;
       const int e = it - a.tau;
-      float s1 = ld(row, e), s0;
+      float sv[MAXW + 1];
 #pragma unroll
-      for (int j = 0; j < SB_MAX_W; j++) {
+      for (int j = 0; j <= MAXW; j++)
+        if (j <= w) sv[j] = ld(row, e + j);
+      float s1 = sv[0], s0;
+#pragma unroll
+      for (int j = 0; j < MAXW; j++) {
         if (j < w) {
           s0 = s1;
-          s1 = ld(row, e + j + 1);
+          s1 = sv[j + 1];
           float d = s1 - s0;
           float v = d * x + s0;
           add(num[j], cnum[j], v);
@@ -275,7 +286,7 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   }
   float acsq = 0.f;
 #pragma unroll
-  for (int j = 0; j < SB_MAX_W; j++)
+  for (int j = 0; j < MAXW; j++)
     if (j < w) {
       float q = num[j] * num[j];
       acsq += q;
@@ -291,7 +302,7 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
 }
 
 // One (CDP, t0, par) of CRS exactly as CRS/OpenMP/src/main.cpp:210-265: double.
-template <class LD>
+template <int MAXW = SB_MAX_W, class LD>
 __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int p, double& sem,
                                double& stk) {
 #pragma clang fp contract(off)
@@ -302,14 +313,18 @@ __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   const double tt0 = a.dt_d * (double)t0;
   const double lo = (double)a.tau, hi = (double)(a.ns - a.tau - 1);
   const int w = a.w;
-  double num[SB_MAX_W];
+  double num[MAXW];
 #pragma unroll
-  for (int j = 0; j < SB_MAX_W; j++) num[j] = 0.0;
+  for (int j = 0; j < MAXW; j++) num[j] = 0.0;
   double den = 0.0, lin = 0.0, mm = 0.0;
   int err = 0;
   const int k0 = a.slot_off[cdp], k1 = a.slot_off[cdp + 1];
+  // Next slot loaded while this one computes; window samples loaded before
+  // the sums (same arithmetic, same order).
+  SlotD s_next = a.slots_d[k0 < k1 ? k0 : 0];
   for (int k = k0; k < k1; k++) {
-    const SlotD s = a.slots_d[k];
+    const SlotD s = s_next;
+    if (k + 1 < k1) s_next = a.slots_d[k + 1];
     double t = tt0 + pa * s.m;
     double tsq = t * t;
     double bm = pb * s.m2;
@@ -322,12 +337,16 @@ __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t
       const int it = (int)ti;
       const double x = ti - (double)it;
       const int e = it - a.tau;
-      double s1 = (double)ld(s.row, e), s0;
+      float sv[MAXW + 1];
 #pragma unroll
-      for (int j = 0; j < SB_MAX_W; j++) {
+      for (int j = 0; j <= MAXW; j++)
+        if (j <= w) sv[j] = ld(s.row, e + j);
+      double s1 = (double)sv[0], s0;
+#pragma unroll
+      for (int j = 0; j < MAXW; j++) {
         if (j < w) {
           s0 = s1;
-          s1 = (double)ld(s.row, e + j + 1);
+          s1 = (double)sv[j + 1];
           double d = s1 - s0;
           double v = d * x + s0;
           num[j] += v;
@@ -343,7 +362,7 @@ __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   }
   double acsq = 0.0;
 #pragma unroll
-  for (int j = 0; j < SB_MAX_W; j++)
+  for (int j = 0; j < MAXW; j++)
     if (j < w) {
       double q = num[j] * num[j];
       acsq += q;
@@ -427,6 +446,9 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
 }
 
 // ------------------------------------------------------------- finalize
+// Winners come from the fast search, which runs windows of at most 7 samples.
+constexpr int kFinalizeMaxW = 7;
+
 // Decodes the packed best key of every (CDP, t0) and re-evaluates the winner
 // in the reference's precision and order; writes the output arrays.
 template <class LD>
@@ -439,19 +461,19 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
   if (a.method == SB_METHOD_CMP) {
     float sem = 0.f, stk = 0.f;
-    if (p >= 0) eval_cmp_exact<LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_cmp_exact<kFinalizeMaxW, LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.f)
       write_cmp(a, o, p, sem, stk);
     else
       write_cmp(a, o, -1, 0.f, 0.f);
   } else {
     double sem = 0.0, stk = 0.0;
-    if (p >= 0) eval_crs_exact(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_crs_exact<kFinalizeMaxW>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.0) {
       write_crs(a, o, p, sem, stk);
     } else {
       double s0;
-      eval_crs_exact(a, ld, cdp, 0, 0, s0, stk);
+      eval_crs_exact<kFinalizeMaxW>(a, ld, cdp, 0, 0, s0, stk);
       write_crs(a, o, 0, 0.0, stk);
     }
   }
