@@ -758,6 +758,119 @@ __global__ __launch_bounds__(256) void k_search_fast(SearchArgs a) {
   search_item<W, R, CC, CRS, STORE>(a, a.work[blockIdx.x], s_best);
 }
 
+// ------------------------------------------- SB_MODE_EXACT CMP, tiled
+// The reference's CMP arithmetic (CMP/OpenMP/src/main.cpp:150-203: float, no
+// contraction, correctly rounded sqrt and division, traces in gather order,
+// the validity test on the float traveltime) in the tiled layout of
+// k_search_fast: one workgroup per (CDP, 64-t0 tile, parameter chunk), each
+// wave a contiguous run of the chunk, CC parameters per pass over the traces,
+// one 16-byte window load per evaluation, and the first-max arg-max as the
+// packed key (semblance bits, ~index).  k_finalize then evaluates each winner
+// with eval_cmp_exact, the same arithmetic, so the outputs equal
+// k_search_exact's.  FP32 store, windows of at most 7 samples.
+template <int W, int CC>
+__global__ __launch_bounds__(256) void k_search_exact_tiled(SearchArgs a) {
+#pragma clang fp contract(off)
+  __shared__ unsigned long long s_best[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;
+  const int cdp = wi.cdp;
+  const int r0 = a.cdp_off[cdp], r1 = a.cdp_off[cdp + 1];
+  const int S = r1 - r0;
+  const int t0 = wi.tile * 64 + lane;
+  float tt0 = a.dt_f * (float)t0;
+  tt0 = tt0 * tt0;
+  const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
+  const int np = wi.p1 - wi.p0;
+  const int per = ((np + 4 * CC - 1) / (4 * CC)) * CC;
+  const int pw0 = wi.p0 + wave * per;
+  const int pw1 = min(pw0 + per, wi.p1);
+  const int nrec = (int)(a.stride * 4);
+  const char* __restrict__ base = (const char*)a.samples;
+  unsigned long long best = 0ull;
+
+  for (int p = pw0; p < pw1; p += CC) {
+    float c[CC];
+    bool act[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      act[ci] = p + ci < pw1;
+      c[ci] = a.cf0 + a.cfinc * (float)(act[ci] ? p + ci : p);
+    }
+    float num[CC][W], den[CC];
+    int cnt[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      den[ci] = 0.f;
+      cnt[ci] = 0;
+#pragma unroll
+      for (int j = 0; j < W; j++) num[ci][j] = 0.f;
+    }
+    float h_next = S > 0 ? a.h_ref32[r0] : 0.f;
+    for (int row = r0; row < r1; row++) {
+      const float h = h_next;
+      if (row + 1 < r1) h_next = a.h_ref32[row + 1];
+      const __amdgpu_buffer_rsrc_t rs =
+          __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)row * a.stride * 4), (short)0, nrec, kRsrcWord3);
+      unsigned off[CC];
+      float xx[CC];
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        const float ch = c[ci] * h;
+        const float t = sqrtf(tt0 + ch) * a.idt_f;
+        const bool ok = t >= lo && t < hi;
+        const int it = ok ? (int)t : 0;
+        const float x = t - (float)it;
+        cnt[ci] += ok ? 1 : 0;
+        xx[ci] = ok ? x : 0.f;  // invalid traces contribute exactly nothing
+        off[ci] = ok ? (unsigned)(it - a.tau) * 4u : kOob;
+      }
+      Win<W + 1> sv[CC];
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) sv[ci] = load_win<W + 1, SB_STORE_F32>(rs, off[ci], 0u);
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const float d = sv[ci].v[j + 1] - sv[ci].v[j];
+          const float v = d * xx[ci] + sv[ci].v[j];
+          num[ci][j] += v;
+          const float vv = v * v;
+          den[ci] += vv;
+        }
+      }
+    }
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      float acsq = 0.f;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const float q = num[ci][j] * num[ci][j];
+        acsq += q;
+      }
+      const float m = (float)cnt[ci];
+      const float wf = (float)W;
+      if (act[ci] && (double)den[ci] > 1e-13 && (double)m > 1e-13 && (double)wf > 1e-13 && S - cnt[ci] < 2) {
+        const float sem = acsq / (den[ci] * m);
+        if (sem > 0.f) {
+          const unsigned long long key =
+              ((unsigned long long)__float_as_uint(sem) << 32) | (unsigned long long)(~(uint32_t)(p + ci));
+          best = key > best ? key : best;
+        }
+      }
+    }
+  }
+  s_best[wave][lane] = best;
+  __syncthreads();
+  if (wave == 0) {
+    unsigned long long m = s_best[0][lane];
+#pragma unroll
+    for (int q = 1; q < 4; q++) m = s_best[q][lane] > m ? s_best[q][lane] : m;
+    if (t0 < a.ns && m) atomicMax(&a.keys[(int64_t)(cdp - a.cdp_base) * a.ns + t0], m);
+  }
+}
+
 // Persistent grid (a separate kernel, so the per-item kernel's register
 // allocation is not shaped by this loop): the blocks of XCD
 // x = blockIdx % 8 (workgroups are dealt to the 8 XCDs round-robin) take the
@@ -852,6 +965,22 @@ hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipS
   if (store == SB_STORE_F16)
     return exact_with(a, LoadF16{(const __half*)a.samples, a.scale, a.stride}, nblocks, st);
   return exact_with(a, LoadBF16{(const uint16_t*)a.samples, a.scale, a.stride}, nblocks, st);
+}
+
+bool exact_tiled_supported(int method, int store, int w) {
+  return method == SB_METHOD_CMP && store == SB_STORE_F32 && (w == 1 || w == 3 || w == 5 || w == 7);
+}
+
+hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  switch (a.w) {
+    case 1: hipLaunchKernelGGL((k_search_exact_tiled<1, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_search_exact_tiled<3, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_search_exact_tiled<5, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_search_exact_tiled<7, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return SB_CHECK_LAUNCH();
 }
 
 template <int W, int R, int CC, bool CRS, int STORE>

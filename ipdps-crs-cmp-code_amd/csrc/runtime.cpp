@@ -673,6 +673,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   // Windows the fast kernel is not compiled for (w > 7) run in reference
   // arithmetic: slower, same parity bar.
   if (mode == SB_MODE_FAST && !fast_supported(a.w)) mode = SB_MODE_EXACT;
+  // Reference arithmetic on the tiled layout (keys + winner re-evaluation)
+  // where it is compiled; otherwise one thread per (CDP, t0).
+  const bool tiled_exact = mode == SB_MODE_EXACT && exact_tiled_supported(a.method, store, a.w);
   if (mode == SB_MODE_FAST) {
     variant = ctx->fast_variant;
     v = kFastVariants[variant];
@@ -695,6 +698,11 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
       const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
       if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
     }
+  } else if (tiled_exact) {
+    const int ntiles = (g->ns + 63) / 64;
+    const int pchunk = choose_pchunk(ctx, ce - cb, ntiles, a.npar, 16);
+    const std::string key = work_key(g, a.method, 2, ntiles, a.npar, pchunk, cb, ce);
+    if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
   } else {
     const int ntiles = (g->ns + 63) / 64;
     const std::string key = work_key(g, a.method, 1, ntiles, 1, 1, cb, ce);
@@ -705,7 +713,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   const int64_t nout = (int64_t)(ce - cb) * g->ns;
   HIPR(record(ctx, 0));
   hipError_t e = hipSuccess;
-  if (mode == SB_MODE_FAST) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
+  if (mode == SB_MODE_FAST || tiled_exact) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fill");
   if (mode == SB_MODE_FAST && !lds && ctx->persistent) {
     // Persistent grid: 8 per-XCD item counters, cleared by a kernel (not a
@@ -720,11 +728,13 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
     e = launch_search_lds(a, store, v, nblocks, ctx->stream);
   else if (mode == SB_MODE_FAST)
     e = launch_search_fast(a, store, variant, nblocks, ctx->num_cus, ctx->stream);
+  else if (tiled_exact)
+    e = launch_search_exact_tiled(a, nblocks, ctx->stream);
   else
     e = launch_search_exact(a, store, nblocks, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "search launch");
   HIPR(record(ctx, 2));
-  if (mode == SB_MODE_FAST) {
+  if (mode == SB_MODE_FAST || tiled_exact) {
     e = launch_finalize(a, store, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "finalize launch");
   }
@@ -732,7 +742,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
-  ctx->last.block_threads = mode == SB_MODE_FAST ? 256 : 64;
+  ctx->last.block_threads = mode == SB_MODE_FAST || tiled_exact ? 256 : 64;
   ctx->last.variant = variant;
   ctx->last.lds_staged = lds ? 1 : 0;
   ctx->timing_pending = true;

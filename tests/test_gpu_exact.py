@@ -1,0 +1,86 @@
+"""SB_MODE_EXACT for CMP equals the oracle (the reference's float arithmetic
+and order) bit for bit, on both of its kernels.
+
+Windows of 1-7 samples on an FP32 store run the tiled search (64-t0 tiles,
+four parameters per trace pass, packed first-max keys, then the winner's
+re-evaluation); wider windows and 16-bit stores run one thread per (CDP, t0).
+Lines with uneven fold, shuffled offsets, late windows off the trace end, a
+grid with negative C (invalid traveltimes) and several chunkings of the
+parameter range."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("index", "c", "coher", "stack")
+
+
+def _line(sb, seed, ns=601, ncdp=10, fold=(2, 70), h_max=2500.0):
+    off, geom, s = sb.synth(0, ncdp, fold, ns, 2000, seed, h_max=h_max, h_jitter=5.0, nevents=5)
+    rng = np.random.default_rng(seed)
+    perm = np.concatenate([off[i] + rng.permutation(off[i + 1] - off[i]) for i in range(len(off) - 1)])
+    return off, geom[perm], np.ascontiguousarray(s[perm])
+
+
+@pytest.mark.parametrize("tau", [0.0, 0.002, 0.004, 0.006, 0.008])
+def test_exact_equals_oracle(sb, orc, gpu_ctx, tau):
+    print("case: exact cmp tau=%g" % tau, flush=True)
+    off, geom, s = _line(sb, 61)
+    c0, c1, nc = 1.0e-7, 2.5e-6, 45
+    ref = orc.cmp(off, geom, s, 2000, c0, c1, nc, tau)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000)
+    try:
+        out = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau, sb.MODE_EXACT))
+        part = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau, sb.MODE_EXACT), cdp_range=(3, 7))
+    finally:
+        g.close()
+    assert (ref["index"] >= 0).any() and (ref["index"] < 0).any()
+    for k in KEYS:
+        assert np.array_equal(out[k], ref[k]), k
+        assert np.array_equal(part[k], ref[k][3:7]), k
+
+
+def test_exact_negative_c_grid(sb, orc, gpu_ctx):
+    print("case: exact cmp negative c", flush=True)
+    off, geom, s = _line(sb, 63, ncdp=6)
+    c0, c1, nc, tau = -1.0e-6, 1.5e-6, 40, 0.002
+    ref = orc.cmp(off, geom, s, 2000, c0, c1, nc, tau)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000)
+    try:
+        fast = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau, sb.MODE_FAST))  # runs exact (c < 0)
+        ex = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau, sb.MODE_EXACT))
+    finally:
+        g.close()
+    for k in KEYS:
+        assert np.array_equal(ex[k], ref[k]), k
+        assert np.array_equal(fast[k], ref[k]), k
+
+
+@pytest.mark.parametrize("store", ["f16", "bf16"])
+def test_exact_on_16bit_store_matches_restored_input(sb, orc, gpu_ctx, store):
+    """16-bit stores run the per-(CDP, t0) kernel on the restored amplitudes:
+    equal to the oracle on the same amplitudes rounded as the store does
+    (power-of-two scale, round to nearest)."""
+    print("case: exact cmp store=%s" % store, flush=True)
+    off, geom, s = _line(sb, 65, ncdp=5)
+    scale = np.ones(len(s), np.float32)
+    mx = np.abs(s).max(axis=1)
+    nz = mx > 0
+    scale[nz] = np.ldexp(np.float32(1.0), np.frexp(mx[nz])[1]).astype(np.float32)
+    q = s / scale[:, None]
+    if store == "f16":
+        q = q.astype(np.float16).astype(np.float32)
+    else:
+        u = q.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000  # round to nearest even, as __float2bfloat16
+        q = u.astype(np.uint32).view(np.float32)
+    restored = (q * scale[:, None]).astype(np.float32)
+    c0, c1, nc, tau = 1.98e-7, 1.77e-6, 33, 0.002
+    ref = orc.cmp(off, geom, restored, 2000, c0, c1, nc, tau)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=sb.STORES[store])
+    try:
+        out = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau, sb.MODE_EXACT))
+    finally:
+        g.close()
+    for k in KEYS:
+        assert np.array_equal(out[k], ref[k]), k
