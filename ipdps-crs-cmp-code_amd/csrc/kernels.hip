@@ -871,6 +871,137 @@ __global__ __launch_bounds__(256) void k_search_exact_tiled(SearchArgs a) {
   }
 }
 
+// ------------------------------------------- SB_MODE_EXACT CRS, tiled
+// The reference's CRS arithmetic (CRS/OpenMP/src/main.cpp:210-285: double, the
+// supergather in list order, the validity test on the double traveltime) in
+// the tiled layout: one workgroup per (CDP, 64-t0 tile) covering every
+// parameter, each wave a contiguous quarter of the grid, CC parameters per
+// pass over the supergather.  A double semblance and its index do not fit one
+// 64-bit key, so each lane keeps its first maximum (strict '>' in parameter
+// order), the waves' maxima are merged in LDS in wave order with the same
+// rule, and the winner's index is written as a key for k_finalize, which
+// re-evaluates it with eval_crs_exact (no winner: key 0, the reference's
+// absolute-index-0 case).  FP32 store, windows of at most 7 samples.
+template <int W, int CC>
+__global__ __launch_bounds__(256) void k_search_exact_crs_tiled(SearchArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double s_sem[4][64];
+  __shared__ int s_idx[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;
+  const int cdp = wi.cdp;
+  const int k0 = a.slot_off[cdp], k1 = a.slot_off[cdp + 1];
+  const int S = k1 - k0;
+  const int t0 = wi.tile * 64 + lane;
+  const double tt0 = a.dt_d * (double)t0;
+  const double lo = (double)a.tau, hi = (double)(a.ns - a.tau - 1);
+  const int np = wi.p1 - wi.p0;
+  const int per = ((np + 4 * CC - 1) / (4 * CC)) * CC;
+  const int pw0 = wi.p0 + wave * per;
+  const int pw1 = min(pw0 + per, wi.p1);
+  const int nbnc = a.nc * a.nb;
+  const int nrec = (int)(a.stride * 4);
+  const char* __restrict__ base = (const char*)a.samples;
+  double best = 0.0;  // the reference starts its maximum at 0
+  int best_p = -1;
+
+  for (int p = pw0; p < pw1; p += CC) {
+    double pa[CC], pb[CC], pc[CC];
+    bool act[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      act[ci] = p + ci < pw1;
+      const int q = act[ci] ? p + ci : p;
+      pa[ci] = a.da0 + a.dinc_a * (double)(q / nbnc);
+      pb[ci] = a.db0 + a.dinc_b * (double)((q / a.nc) % a.nb);
+      pc[ci] = a.dc0 + a.dinc_c * (double)(q % a.nc);
+    }
+    double num[CC][W], den[CC];
+    int cnt[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      den[ci] = 0.0;
+      cnt[ci] = 0;
+#pragma unroll
+      for (int j = 0; j < W; j++) num[ci][j] = 0.0;
+    }
+    SlotD s_next = a.slots_d[S > 0 ? k0 : 0];
+    for (int k = k0; k < k1; k++) {
+      const SlotD sd = s_next;
+      if (k + 1 < k1) s_next = a.slots_d[k + 1];
+      const __amdgpu_buffer_rsrc_t rs =
+          __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)sd.row * a.stride * 4), (short)0, nrec, kRsrcWord3);
+      unsigned off[CC];
+      double xx[CC];
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        double t = tt0 + pa[ci] * sd.m;
+        const double tsq = t * t;
+        const double bm = pb[ci] * sd.m2;
+        const double ch = pc[ci] * sd.h;
+        t = tsq + bm;
+        t = t + ch;
+        t = t < 0.0 ? -1.0 : sqrt(t);
+        const double ti = t * a.idt_d;
+        const bool ok = ti >= lo && ti < hi;
+        const int it = ok ? (int)ti : 0;
+        cnt[ci] += ok ? 1 : 0;
+        xx[ci] = ok ? ti - (double)it : 0.0;  // invalid traces contribute exactly nothing
+        off[ci] = ok ? (unsigned)(it - a.tau) * 4u : kOob;
+      }
+      Win<W + 1> sv[CC];
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) sv[ci] = load_win<W + 1, SB_STORE_F32>(rs, off[ci], 0u);
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        double s1 = (double)sv[ci].v[0], s0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          s0 = s1;
+          s1 = (double)sv[ci].v[j + 1];
+          const double d = s1 - s0;
+          const double v = d * xx[ci] + s0;
+          num[ci][j] += v;
+          const double vv = v * v;
+          den[ci] += vv;
+        }
+      }
+    }
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      double acsq = 0.0;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const double q = num[ci][j] * num[ci][j];
+        acsq += q;
+      }
+      const double mm = (double)cnt[ci];
+      double sem = 0.0;
+      if (den[ci] > 1e-13 && mm > 1e-13 && (double)W > 1e-13 && S - cnt[ci] < 2) sem = acsq / (den[ci] * mm);
+      if (act[ci] && sem > best) {
+        best = sem;
+        best_p = p + ci;
+      }
+    }
+  }
+  s_sem[wave][lane] = best;
+  s_idx[wave][lane] = best_p;
+  __syncthreads();
+  if (wave == 0 && t0 < a.ns) {
+    double m = s_sem[0][lane];
+    int mp = s_idx[0][lane];
+#pragma unroll
+    for (int q = 1; q < 4; q++)
+      if (s_sem[q][lane] > m) {  // later waves hold later parameters: strict '>' keeps the first maximum
+        m = s_sem[q][lane];
+        mp = s_idx[q][lane];
+      }
+    a.keys[(int64_t)(cdp - a.cdp_base) * a.ns + t0] =
+        mp >= 0 ? ((1ull << 32) | (unsigned long long)(~(uint32_t)mp)) : 0ull;
+  }
+}
+
 // Persistent grid (a separate kernel, so the per-item kernel's register
 // allocation is not shaped by this loop): the blocks of XCD
 // x = blockIdx % 8 (workgroups are dealt to the 8 XCDs round-robin) take the
@@ -968,11 +1099,21 @@ hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipS
 }
 
 bool exact_tiled_supported(int method, int store, int w) {
-  return method == SB_METHOD_CMP && store == SB_STORE_F32 && (w == 1 || w == 3 || w == 5 || w == 7);
+  return store == SB_STORE_F32 && (w == 1 || w == 3 || w == 5 || w == 7);
 }
 
 hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream_t st) {
   if (nblocks <= 0) return hipSuccess;
+  if (a.method == SB_METHOD_CRS) {
+    switch (a.w) {
+      case 1: hipLaunchKernelGGL((k_search_exact_crs_tiled<1, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 3: hipLaunchKernelGGL((k_search_exact_crs_tiled<3, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 5: hipLaunchKernelGGL((k_search_exact_crs_tiled<5, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 7: hipLaunchKernelGGL((k_search_exact_crs_tiled<7, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return SB_CHECK_LAUNCH();
+  }
   switch (a.w) {
     case 1: hipLaunchKernelGGL((k_search_exact_tiled<1, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
     case 3: hipLaunchKernelGGL((k_search_exact_tiled<3, 4>), dim3(nblocks), dim3(256), 0, st, a); break;

@@ -41,9 +41,11 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, int num_cus, hipStream_t st);
 hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& v, int nblocks, hipStream_t st);
 hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipStream_t st);
-// Reference-arithmetic CMP search in the tiled layout (work items as the fast
-// search's with 64-t0 tiles; keys, then launch_finalize).  CMP, FP32 store,
-// w in {1, 3, 5, 7}.
+// Reference-arithmetic search in the tiled layout (work items as the fast
+// search's with 64-t0 tiles; keys, then launch_finalize).  FP32 store, w in
+// {1, 3, 5, 7}.  CMP: any parameter chunking (first-max keys, atomicMax);
+// CRS: one work item per (CDP, tile) covering every parameter (the double
+// semblances are compared in the workgroup, not through keys).
 bool exact_tiled_supported(int method, int store, int w);
 hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);

@@ -700,7 +700,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
     }
   } else if (tiled_exact) {
     const int ntiles = (g->ns + 63) / 64;
-    const int pchunk = choose_pchunk(ctx, ce - cb, ntiles, a.npar, 16);
+    const int pchunk = a.method == SB_METHOD_CRS ? a.npar : choose_pchunk(ctx, ce - cb, ntiles, a.npar, 16);
     const std::string key = work_key(g, a.method, 2, ntiles, a.npar, pchunk, cb, ce);
     if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
   } else {

@@ -84,3 +84,33 @@ def test_exact_on_16bit_store_matches_restored_input(sb, orc, gpu_ctx, store):
         g.close()
     for k in KEYS:
         assert np.array_equal(out[k], ref[k]), k
+
+
+def _crs_line(sb, seed, ncdp=6):
+    off, geom, s = sb.synth(1, ncdp, (20, 60), 401, 4000, seed, h_max=800.0, mid_half_width=25.0, nevents=4)
+    no, nb = sb.neighbors(geom[off[:-1]], 30)
+    return off, geom, s, no, nb
+
+
+@pytest.mark.parametrize("tau", [0.0, 0.004, 0.008, 0.012, 0.016])
+def test_crs_exact_equals_oracle(sb, orc, gpu_ctx, tau):
+    """CRS in the reference's double arithmetic: the tiled search (windows
+    up to 7 samples, one workgroup per CDP and tile, first maximum merged
+    across waves) and the per-(CDP, t0) search (wider windows) equal the
+    oracle bit for bit, including t0 with no positive semblance (index 0 and
+    the stack of (t0 = 0, parameter 0))."""
+    print("case: exact crs tau=%g" % tau, flush=True)
+    off, geom, s, no, nb = _crs_line(sb, 71)
+    box = (-0.7e-3, 0.7e-3, 5, -1e-7, 1e-7, 4, 1.975e-7, 1.384e-6, 6)
+    ref = orc.crs(off, geom, s, 4000, no, nb, *box, tau)
+    g = sb.Gather(gpu_ctx, off, geom, s, 4000, nbr_off=no, nbr=nb)
+    try:
+        prm = sb.crs_params(box[0], box[1], box[2], box[3], box[4], box[5], box[6], box[7], box[8], tau,
+                            sb.MODE_EXACT)
+        out = sb.search(gpu_ctx, g, prm)
+        part = sb.search(gpu_ctx, g, prm, cdp_range=(2, 5))
+    finally:
+        g.close()
+    for k in ("index", "a", "b", "c", "coher", "stack"):
+        assert np.array_equal(out[k], ref[k]), k
+        assert np.array_equal(part[k], ref[k][2:5]), k
