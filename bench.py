@@ -332,7 +332,7 @@ def side_configs(sb, args):
     return out
 
 
-def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4):
+def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, store=0):
     """Serving loop under load.  A CMP line (96 CDPs of fold 24-1000, kept at
     ns 501/1001/1501/2001/3001) and a CRS line (32 CDPs, ns 501/1001) are made
     resident on the server once; requests name a line and a CDP range.
@@ -374,10 +374,11 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4):
     with sb.Server(0, max_batch=64) as srv:
         gid = {}
         for n in (501, 1001, 1501, 2001, 3001):
-            gid[("cmp", n)] = srv.add_gather(cmp_off, cmp_geom, np.ascontiguousarray(cmp_s[:, :n]), 2000)
+            gid[("cmp", n)] = srv.add_gather(cmp_off, cmp_geom, np.ascontiguousarray(cmp_s[:, :n]), 2000,
+                                             store=store)
         for n in (501, 1001):
             gid[("crs", n)] = srv.add_gather(crs_off, crs_geom, np.ascontiguousarray(crs_s[:, :n]), 4000,
-                                             nbr_off=crs_no, nbr=crs_nb)
+                                             store=store, nbr_off=crs_no, nbr=crs_nb)
         resident = srv.stats()["hbm_reserved"]
         # warm-up: one request of each kind
         for ev in events[:4]:
@@ -396,7 +397,8 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4):
                 lat[kind].append(st["t_done_ms"] - st["t_submit_ms"])
         wall = time.perf_counter() - start
         stats = srv.stats()
-    res = dict(offered_rps=rate + probe_rate, duration_s=round(wall, 3), served=stats["done"] - 4,
+    res = dict(store="f32" if store == 0 else "f16", offered_rps=rate + probe_rate, duration_s=round(wall, 3),
+               served=stats["done"] - 4,
                launches=stats["launches"] - 4, resident_gb=round(resident / 1e9, 2))
     for k, v in lat.items():
         v = np.array(v)
@@ -553,6 +555,7 @@ def main():
         line["cli_config0"] = cli_config0(sb)
     if not args.no_serving:
         line["serving"] = serving_bench(sb)
+        line["serving_fp16_store"] = serving_bench(sb, store=sb.STORE_F16)
         if "probe" in line["serving"]:
             line["panel_latency_ms"]["p99_under_load"] = line["serving"]["probe"]["p99_ms"]
     print(json.dumps(line), flush=True)
