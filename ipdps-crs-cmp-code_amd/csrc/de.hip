@@ -143,13 +143,15 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   // this every wave waits out each load's latency).  Same order of sums.
   // FP16 windows stay halves and feed mixed-precision FMAs (v_fma_mix_f32);
   // BF16 and FP32 windows are FP32.
+  const float T0sq = __fmul_rn(T0, T0);
+  const float K1 = __fmul_rn(__fmul_rn(2.f, T0), pa), K2 = fmaf(pa, pa, pb);
   constexpr bool HALF = STORE == SB_STORE_F16;
   typedef typename std::conditional<HALF, WinH<W + 1>, Win<W + 1>>::type WinT;
   auto issue = [&](const SlotF& s, WinT& sv, float& x, bool& ok) {
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
-    const float y = fmaf(pa, s.m, T0);
-    const float tt = fmaf(y, y, fmaf(pb, s.m2, pc * s.h));
+    // (T0 + a m)^2 + b m^2 + c h expanded: three FMAs per trace
+    const float tt = fmaf(K2, s.m2, fmaf(K1, s.m, fmaf(pc, s.h, T0sq)));
     // t^2 < 0 is invalid (the reference's t = -1): clamp to keep the
     // fraction finite; the window test rejects it unless tau = 0.
     const float ts = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.f));

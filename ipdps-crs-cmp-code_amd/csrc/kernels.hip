@@ -544,6 +544,15 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
         pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)qq));
       }
     }
+    // CRS: t^2 = T0^2 + (2 T0 a) m + (a^2 + b) m^2 + c h, the per-(row,
+    // parameter) coefficients formed once per pass over the traces.
+    float K1[R][CC], K2[CC];
+#pragma unroll
+    for (int ci = 0; ci < CC; ci++) {
+      K2[ci] = CRS ? fmaf(pa[ci], pa[ci], pb[ci]) : 0.f;
+#pragma unroll
+      for (int r = 0; r < R; r++) K1[r][ci] = CRS ? __fmul_rn(__fmul_rn(2.f, T0[r]), pa[ci]) : 0.f;
+    }
     float num[R][CC][W], den[R][CC];
     int cnt[R][CC];
 #pragma unroll
@@ -570,14 +579,6 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
           (void*)(base + (int64_t)s.row * a.stride * ESZ - lead), (short)0, nrec + lead, kRsrcWord3);
       // Every rounding step below is explicit (fmaf / __fmul_rn), so an
       // evaluation's bits do not depend on the launch shape or batch.
-      float U[CC];
-#pragma unroll
-      for (int ci = 0; ci < CC; ci++) {
-        if constexpr (CRS)
-          U[ci] = fmaf(pb[ci], s.m2, __fmul_rn(pc[ci], s.h));
-        else
-          U[ci] = 0.f;
-      }
       unsigned off[R][CC], shf[R][CC];
       float xx[R][CC];
 #pragma unroll
@@ -586,8 +587,8 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
         for (int ci = 0; ci < CC; ci++) {
           float tt;
           if constexpr (CRS) {
-            const float y = fmaf(pa[ci], s.m, T0[r]);
-            tt = fmaf(y, y, U[ci]);
+            // (T0 + a m)^2 + b m^2 + c h expanded: three FMAs per evaluation
+            tt = fmaf(K2[ci], s.m2, fmaf(K1[r][ci], s.m, fmaf(pc[ci], s.h, T0sq[r])));
           } else {
             tt = fmaf(pc[ci], s.h, T0sq[r]);
           }
