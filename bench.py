@@ -500,10 +500,11 @@ def main():
                 print("bench: %s launch modes %s" % (store, results[store]["modes"]), flush=True)
             g.close()
         p50, p99 = (None, None) if args.no_latency else panel_latency(sb, ctx, cdp_off, geom, samples)
-    if "f32" not in results or "f16" not in results:
+    if "f32" not in results:
         print(json.dumps({k: v for k, v in results.items()}), flush=True)
         return
     f32 = results["f32"]
+    f16 = results.get("f16")
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
     skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
     traffic = None
@@ -528,10 +529,10 @@ def main():
                    "ncdp": CFG["ncdp"], "ns": CFG["ns"], "nc": CFG["nc"], "tuples_per_step": f32["tuples"],
                    "launch": {"variant": f32["tune"][0], "pchunk": f32["tune"][1], "blocks": f32["blocks"],
                               "lds_staged": f32["lds_staged"]}},
-        "fp16_store": {"value": results["f16"]["value"], "ms_per_step": results["f16"]["ms_per_step"],
-                       "speedup_vs_f32": results["f16"]["value"] / f32["value"],
-                       "launch": {"variant": results["f16"]["tune"][0], "pchunk": results["f16"]["tune"][1]}},
-        "launch_modes_ms": {k: results[k].get("modes") for k in ("f32", "f16")},
+        "fp16_store": None if f16 is None else {
+            "value": f16["value"], "ms_per_step": f16["ms_per_step"], "speedup_vs_f32": f16["value"] / f32["value"],
+            "launch": {"variant": f16["tune"][0], "pchunk": f16["tune"][1]}},
+        "launch_modes_ms": {k: results[k].get("modes") for k in results},
         "panel_latency_ms": {"p50": p50, "p99": p99, "what": "1-CDP CMP panel, upload+search+download, sequential"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VECTOR_TFLOPS, "traffic": traffic,
