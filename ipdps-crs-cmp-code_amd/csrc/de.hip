@@ -133,6 +133,7 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
   const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
   const int nrec = (int)(a.stride * ESZ);
+  const uint32_t row_bytes = (uint32_t)(a.stride * ESZ);
   const char* __restrict__ base = (const char*)a.samples;
   float num[W], den = 0.f;
   int cnt = 0;
@@ -149,7 +150,7 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   typedef typename std::conditional<HALF, WinH<W + 1>, Win<W + 1>>::type WinT;
   auto issue = [&](const SlotF& s, WinT& sv, float& x, bool& ok) {
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)s.row * a.stride * ESZ), (short)0, nrec, kRsrcWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void*)(base + (uint64_t)(uint32_t)s.row * row_bytes), (short)0, nrec, kRsrcWord3);
     // (T0 + a m)^2 + b m^2 + c h expanded: three FMAs per trace
     const float tt = fmaf(K2, s.m2, fmaf(K1, s.m, fmaf(pc, s.h, T0sq)));
     // t^2 < 0 is invalid (the reference's t = -1): clamp to keep the
@@ -196,20 +197,26 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
       den = fmaf(s.scale2, q, den);
     }
   };
+  // Slots are read two traces ahead of their use, so the scalar loads are in
+  // flight while a trace accumulates (indices clamped to the last trace; a
+  // repeat is loaded, never summed).
   if (k0 < k1) {
     WinT wa, wb;
     float xa, xb;
     bool oa, ob;
     SlotF sa = a.slots[k0];
+    SlotF sb = a.slots[k0 + 1 < k1 ? k0 + 1 : k1 - 1];
     issue(sa, wa, xa, oa);
     for (int k = k0; k < k1; k += 2) {
-      const SlotF sb = a.slots[k + 1 < k1 ? k + 1 : k];
-      issue(sb, wb, xb, ob);  // a repeat of trace k at the end: loaded, never summed
+      const SlotF sc = a.slots[k + 2 < k1 ? k + 2 : k1 - 1];
+      issue(sb, wb, xb, ob);
       consume(sa, wa, xa, oa);
       if (k + 1 >= k1) break;
-      sa = a.slots[k + 2 < k1 ? k + 2 : k + 1];
-      issue(sa, wa, xa, oa);
+      const SlotF sd = a.slots[k + 3 < k1 ? k + 3 : k1 - 1];
+      issue(sc, wa, xa, oa);
       consume(sb, wb, xb, ob);
+      sa = sc;
+      sb = sd;
     }
   }
   float acsq = 0.f, lin = 0.f;
