@@ -447,12 +447,14 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
 }
 
 // ------------------------------------------------------------- finalize
-// Winners come from the fast search, which runs windows of at most 7 samples.
+// Winners of windows up to 7 samples (the fast search's) are re-evaluated by
+// the kFinalizeMaxW instantiation; wider windows (tiled reference-arithmetic
+// search only) by the SB_MAX_W one.
 constexpr int kFinalizeMaxW = 7;
 
 // Decodes the packed best key of every (CDP, t0) and re-evaluates the winner
 // in the reference's precision and order; writes the output arrays.
-template <class LD>
+template <int MAXW, class LD>
 __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int64_t n = (int64_t)a.nout * a.ns;
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,19 +464,19 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
   if (a.method == SB_METHOD_CMP) {
     float sem = 0.f, stk = 0.f;
-    if (p >= 0) eval_cmp_exact<kFinalizeMaxW, LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_cmp_exact<MAXW, LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.f)
       write_cmp(a, o, p, sem, stk);
     else
       write_cmp(a, o, -1, 0.f, 0.f);
   } else {
     double sem = 0.0, stk = 0.0;
-    if (p >= 0) eval_crs_exact<kFinalizeMaxW>(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_crs_exact<MAXW>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.0) {
       write_crs(a, o, p, sem, stk);
     } else {
       double s0;
-      eval_crs_exact<kFinalizeMaxW>(a, ld, cdp, 0, 0, s0, stk);
+      eval_crs_exact<MAXW>(a, ld, cdp, 0, 0, s0, stk);
       write_crs(a, o, 0, 0.0, stk);
     }
   }
@@ -1071,7 +1073,10 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 template <class LD>
 static hipError_t finalize_with(const SearchArgs& a, const LD& ld, hipStream_t st) {
   const int64_t n = (int64_t)a.nout * a.ns;
-  hipLaunchKernelGGL(k_finalize<LD>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
+  if (a.w <= kFinalizeMaxW)
+    hipLaunchKernelGGL((k_finalize<kFinalizeMaxW, LD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
+  else
+    hipLaunchKernelGGL((k_finalize<SB_MAX_W, LD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
   return SB_CHECK_LAUNCH();
 }
 
@@ -1100,7 +1105,7 @@ hipError_t launch_search_exact(const SearchArgs& a, int store, int nblocks, hipS
 }
 
 bool exact_tiled_supported(int method, int store, int w) {
-  return store == SB_STORE_F32 && (w == 1 || w == 3 || w == 5 || w == 7);
+  return store == SB_STORE_F32 && w >= 1 && w <= SB_MAX_W && (w & 1);
 }
 
 hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream_t st) {
@@ -1111,6 +1116,10 @@ hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream
       case 3: hipLaunchKernelGGL((k_search_exact_crs_tiled<3, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
       case 5: hipLaunchKernelGGL((k_search_exact_crs_tiled<5, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
       case 7: hipLaunchKernelGGL((k_search_exact_crs_tiled<7, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 9: hipLaunchKernelGGL((k_search_exact_crs_tiled<9, 1>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 11: hipLaunchKernelGGL((k_search_exact_crs_tiled<11, 1>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 13: hipLaunchKernelGGL((k_search_exact_crs_tiled<13, 1>), dim3(nblocks), dim3(256), 0, st, a); break;
+      case 15: hipLaunchKernelGGL((k_search_exact_crs_tiled<15, 1>), dim3(nblocks), dim3(256), 0, st, a); break;
       default: return hipErrorInvalidValue;
     }
     return SB_CHECK_LAUNCH();
@@ -1120,6 +1129,10 @@ hipError_t launch_search_exact_tiled(const SearchArgs& a, int nblocks, hipStream
     case 3: hipLaunchKernelGGL((k_search_exact_tiled<3, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
     case 5: hipLaunchKernelGGL((k_search_exact_tiled<5, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
     case 7: hipLaunchKernelGGL((k_search_exact_tiled<7, 4>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 9: hipLaunchKernelGGL((k_search_exact_tiled<9, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 11: hipLaunchKernelGGL((k_search_exact_tiled<11, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 13: hipLaunchKernelGGL((k_search_exact_tiled<13, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
+    case 15: hipLaunchKernelGGL((k_search_exact_tiled<15, 2>), dim3(nblocks), dim3(256), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return SB_CHECK_LAUNCH();

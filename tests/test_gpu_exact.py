@@ -1,9 +1,10 @@
-"""SB_MODE_EXACT for CMP equals the oracle (the reference's float arithmetic
-and order) bit for bit, on both of its kernels.
+"""SB_MODE_EXACT equals the oracle (the reference's arithmetic and order)
+bit for bit, on both of its kernels.
 
-Windows of 1-7 samples on an FP32 store run the tiled search (64-t0 tiles,
-four parameters per trace pass, packed first-max keys, then the winner's
-re-evaluation); wider windows and 16-bit stores run one thread per (CDP, t0).
+On an FP32 store every window (1-15 samples) runs the tiled search (64-t0
+tiles, several parameters per trace pass, first maxima as packed keys for
+CMP or merged in the workgroup for CRS, then the winner's re-evaluation);
+16-bit stores run one thread per (CDP, t0).
 Lines with uneven fold, shuffled offsets, late windows off the trace end, a
 grid with negative C (invalid traveltimes) and several chunkings of the
 parameter range."""
@@ -22,7 +23,7 @@ def _line(sb, seed, ns=601, ncdp=10, fold=(2, 70), h_max=2500.0):
     return off, geom[perm], np.ascontiguousarray(s[perm])
 
 
-@pytest.mark.parametrize("tau", [0.0, 0.002, 0.004, 0.006, 0.008])
+@pytest.mark.parametrize("tau", [0.0, 0.002, 0.004, 0.006, 0.008, 0.014])
 def test_exact_equals_oracle(sb, orc, gpu_ctx, tau):
     print("case: exact cmp tau=%g" % tau, flush=True)
     off, geom, s = _line(sb, 61)
@@ -92,13 +93,12 @@ def _crs_line(sb, seed, ncdp=6):
     return off, geom, s, no, nb
 
 
-@pytest.mark.parametrize("tau", [0.0, 0.004, 0.008, 0.012, 0.016])
+@pytest.mark.parametrize("tau", [0.0, 0.004, 0.008, 0.012, 0.016, 0.028])
 def test_crs_exact_equals_oracle(sb, orc, gpu_ctx, tau):
-    """CRS in the reference's double arithmetic: the tiled search (windows
-    up to 7 samples, one workgroup per CDP and tile, first maximum merged
-    across waves) and the per-(CDP, t0) search (wider windows) equal the
-    oracle bit for bit, including t0 with no positive semblance (index 0 and
-    the stack of (t0 = 0, parameter 0))."""
+    """CRS in the reference's double arithmetic: the tiled search (one
+    workgroup per CDP and tile, first maximum merged across waves) equals
+    the oracle bit for bit for windows of 1-15 samples, including t0 with no
+    positive semblance (index 0 and the stack of (t0 = 0, parameter 0))."""
     print("case: exact crs tau=%g" % tau, flush=True)
     off, geom, s, no, nb = _crs_line(sb, 71)
     box = (-0.7e-3, 0.7e-3, 5, -1e-7, 1e-7, 4, 1.975e-7, 1.384e-6, 6)
