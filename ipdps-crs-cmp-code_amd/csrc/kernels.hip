@@ -483,6 +483,15 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
 }
 
 // ---------------------------------------------------- SB_MODE_FAST search
+// (x << S) + y in one v_lshl_add_u32 (the compiler would reassociate it into
+// an add and a shift).  A VALU instruction only.
+template <int S>
+__device__ __forceinline__ unsigned lshl_add(int x, unsigned y) {
+  unsigned d;
+  asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(d) : "v"(x), "i"(S), "s"(y));
+  return d;
+}
+
 // One work item (CDP, t0 tile, parameter chunk) by one block of 4 waves.
 template <int W, int R, int CC, bool CRS, int STORE>
 __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem wi,
@@ -526,6 +535,8 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
   // second-to-last trace's test, whatever its windows read), and only the
   // last trace is gated.  Sums, order and validity equal the general path's.
   const bool prefix = !CRS && wi.tile * 64 * R > a.tau;
+  const int tau_even = (a.tau + 1) & ~1, even_adj = tau_even - a.tau;
+  const unsigned adj2 = 2u * (unsigned)even_adj, adj16 = 16u * (unsigned)even_adj;
 
   for (int p = pw0; p < pw1; p += CC) {
     float pa[CC], pb[CC], pc[CC];
@@ -573,10 +584,13 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
     // last trace).
     auto trace = [&](const SlotF& s, auto gate_tag) {
       constexpr int GATE = decltype(gate_tag)::value;
-      // FP32 rows are addressed from tau samples before their start (the
-      // allocation has kSamplesLead bytes in front of row 0), so a window's
-      // byte offset is 4 * floor(t).
-      const int lead = STORE == SB_STORE_F32 ? 4 * a.tau : 0;
+      // Rows are addressed from `lead` bytes before their start (the
+      // allocation has kSamplesLead bytes in front of row 0): FP32 from tau
+      // samples, so a window's byte offset is 4 * floor(t); 16-bit stores
+      // from the even sample count tau_even = tau rounded up to even, so the
+      // window start floor(t) - tau + tau_even keeps the parity of
+      // floor(t) - tau and its dword-aligned half needs no subtraction.
+      const int lead = STORE == SB_STORE_F32 ? 4 * a.tau : 2 * tau_even;
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           (void*)(base + (int64_t)s.row * a.stride * ESZ - lead), (short)0, nrec + lead, kRsrcWord3);
       // Every rounding step below is explicit (fmaf / __fmul_rn), so an
@@ -620,8 +634,11 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
           if constexpr (STORE == SB_STORE_F32) {
             off[r][ci] = ok ? (unsigned)it * 4u : kOob;
           } else {
-            off[r][ci] = ok ? ((unsigned)e & ~1u) * 2u : kOob;
-            shf[r][ci] = ((unsigned)e & 1u) << 4;
+            // e2 = window start + tau_even - tau = it + even_adj:
+            // byte offset (2 e2) & ~3 and funnel shift 16 e2 (low 5 bits:
+            // 16 (e2 & 1)), each shift-and-add one v_lshl_add_u32
+            off[r][ci] = ok ? lshl_add<1>(it, adj2) & ~3u : kOob;
+            shf[r][ci] = lshl_add<4>(it, adj16);
           }
         }
       }
