@@ -16,9 +16,12 @@ struct FastVariant {
 };
 
 // LDS-staged CMP search: at most this many dynamic LDS bytes per block and
-// traces per CDP (the per-trace slab table lives in the same LDS).
+// traces per CDP (the per-trace tables live in the same LDS).
 constexpr int kLdsMaxBytes = 64 * 1024;
 constexpr int kLdsMaxTraces = 2048;
+// Byte offset of the slabs in an LDS-staged block for a CDP of S traces
+// (-1: no such shape); the slabs hold FP32 samples.
+int lds_slab_offset(const FastVariant& v, int S);
 extern const FastVariant kFastVariants[];
 extern const int kNumFastVariants;
 

@@ -1,28 +1,32 @@
 // LDS-staged CMP search (k_search_lds).
 //
-// Same work items and the same arithmetic as k_search_fast (lanes = 64
-// consecutive t0 of one CDP, every lane walks the CDP's traces in order,
-// packed 64-bit arg-max keys), but the sample windows come from LDS instead
-// of the vector L1, whose address path bounds the global kernel (~17.5
-// CU-cycles per 64-lane window load).  A block works through its parameter
-// range in rounds of 4*CC consecutive parameters (CC per wave).  Per round it
-// computes, for every trace, the range of window starts its t0 span and the
-// round's c range can reach (the CMP traveltime sqrt(t0^2 + c h) is monotone
-// in t0 and in c >= 0), packs the traces into chunks that fit the LDS, and
-// for each chunk stages exactly those samples and runs the trace loop from
-// LDS; the windowed sums stay in registers across chunks.
+// Same work items, evaluations and 64-bit arg-max keys as k_search_fast
+// (lanes = 64 consecutive t0 of one CDP, every lane walks the CDP's traces in
+// slot order by increasing h), but each window is read from LDS: the global
+// kernel spends ~16.6 CU-cycles of the vector-memory address path per 64
+// window loads, two aligned ds_read2_b32 cost about half of that and leave
+// the loop bound by its VALU work.
 //
-// Slab layout ("pair layout"): pair i of a trace holds samples (lo+i,
-// lo+i+1), so the W+1 samples of any window start sit in pairs e-lo, e-lo+2,
-// ... -- aligned ds_read2 reads at any start parity.  FP32: 8-byte pairs;
-// FP16/BF16: 4-byte pairs (two halves, interpolated with mixed-precision
-// FMAs).  Pairs 0..W-1 are zero: invalid windows (outside [tau, ns-tau-1))
-// read them, as the global kernel's out-of-range buffer loads read zeros.
+// A block works through its parameter range in rounds of NW * CC consecutive
+// parameters (CC per wave).  Per round it bounds, for every trace, the window
+// starts that any of its lanes can reach: e = floor(sqrt(T0^2 + c h)) - tau is
+// monotone in T0 and in c >= 0, so the block's first t0 with the round's
+// smallest c and its last t0 (t0min + 64R - 1, past ns for the last tile, so
+// that every lane's reads stay inside the staged range) with the largest c
+// bound it, widened by 2 for the approximate square root.  The traces' ranges
+// are packed into chunks that fit the LDS; each chunk is staged as FP32 (the
+// stored bits converted exactly: 16-bit stores keep their per-trace scale
+// for the arithmetic) and the trace loop runs from LDS, the windowed sums
+// staying in registers across chunks.  A window start e of trace k lives at
+// byte address base_k + 4 (e + tau) (one v_lshl_add_u32 per evaluation);
+// gated windows read the zeros at address 0.
 //
-// The values staged are the stored bits and every rounding step matches
-// k_search_fast, in the same trace order, so both kernels produce identical
-// keys.
+// Every rounding step matches k_search_fast, in the same trace order, with
+// the same general/prefix split, last-trace gate and second-to-last-trace
+// validity, so both kernels produce identical keys.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "device_types.hpp"
@@ -32,95 +36,69 @@
 namespace sb {
 
 namespace {
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+// (x << 2) + y in one v_lshl_add_u32, y in a VGPR.
+__device__ __forceinline__ unsigned lshl2_add(int x, unsigned y) {
+  unsigned d;
+  asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(d) : "v"(x), "v"(y));
+  return d;
+}
+// (x << 1) + y and (x << 4) + y, y uniform (the compiler would reassociate
+// them into an add and a shift).
+__device__ __forceinline__ unsigned lshl1_add(int x, unsigned y) {
+  unsigned d;
+  asm("v_lshl_add_u32 %0, %1, 1, %2" : "=v"(d) : "v"(x), "s"(y));
+  return d;
+}
+__device__ __forceinline__ unsigned lshl4_add(int x, unsigned y) {
+  unsigned d;
+  asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(d) : "v"(x), "s"(y));
+  return d;
+}
 
-// One window's raw words: W+1 FP32 samples, or (W+1)/2 dwords of two halves.
-template <int W, int STORE>
-struct RawWin {
-  static constexpr int NP = (W + 1) / 2;
-  float f[STORE == SB_STORE_F32 ? W + 1 : 1];
-  uint32_t d[STORE == SB_STORE_F32 ? 1 : NP];
+// LDS by byte address (the dynamic LDS starts at address 0): the window
+// reads take the computed address as is.
+template <class T>
+__device__ __forceinline__ T lds_at(unsigned addr) {
+  return *(const __attribute__((address_space(3))) T*)(size_t)addr;
+}
+
+// One trace of the CDP as the trace loop reads it (one 16-byte LDS read).
+struct TraceRec {
+  float h;
+  unsigned base;  // LDS byte address of window start -tau in the current chunk
+  float scale, scale2;
 };
 
-// Reads the window whose first pair is `idx` (pair layout: pairs idx,
-// idx+2, ... hold samples 0-1, 2-3, ...).
-template <int W, int STORE>
-__device__ __forceinline__ void read_win(const unsigned char* pairs, unsigned idx, RawWin<W, STORE>& w) {
-  constexpr int NP = (W + 1) / 2;
-  if constexpr (STORE == SB_STORE_F32) {
-    const f2v* pp = (const f2v*)(pairs + (size_t)idx * 8);
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-      const f2v v = pp[2 * j];
-      w.f[2 * j] = v.x;
-      w.f[2 * j + 1] = v.y;
-    }
-  } else {
-    const uint32_t* pp = (const uint32_t*)(pairs + (size_t)idx * 4);
-#pragma unroll
-    for (int j = 0; j < NP; j++) w.d[j] = pp[2 * j];
-  }
-}
-
-// k_search_fast's window arithmetic, operation for operation.
-template <int W, int STORE>
-__device__ __forceinline__ void acc_win(const RawWin<W, STORE>& w, float x, float scale, float scale2, float* num,
-                                        float& den) {
-  constexpr int NP = (W + 1) / 2;
-  if constexpr (STORE == SB_STORE_F32) {
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-      const float v = fmaf(__fsub_rn(w.f[j + 1], w.f[j]), x, w.f[j]);
-      num[j] = __fadd_rn(num[j], v);
-      den = fmaf(v, v, den);
-    }
-  } else if constexpr (STORE == SB_STORE_F16) {
-    _Float16 hv[W + 1];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-      const h2v t = __builtin_bit_cast(h2v, w.d[j]);
-      hv[2 * j] = t.x;
-      hv[2 * j + 1] = t.y;
-    }
-    float q = 0.f;
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-      const float hj = (float)hv[j], hj1 = (float)hv[j + 1];
-      const float v = fmaf(-x, hj, fmaf(x, hj1, hj));
-      num[j] = fmaf(scale, v, num[j]);
-      q = fmaf(v, v, q);
-    }
-    den = fmaf(scale2, q, den);
-  } else {
-    float f[W + 1];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-      f[2 * j] = __uint_as_float(w.d[j] << 16);
-      f[2 * j + 1] = __uint_as_float(w.d[j] & 0xffff0000u);
-    }
-    float q = 0.f;
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-      const float v = fmaf(__fsub_rn(f[j + 1], f[j]), x, f[j]);
-      num[j] = fmaf(scale, v, num[j]);
-      q = fmaf(v, v, q);
-    }
-    den = fmaf(scale2, q, den);
-  }
-}
+// One evaluation in flight: its window and interpolation fraction.
+template <int W>
+struct Ev {
+  float v[W + 1];
+  float x;
+  bool ok;
+};
 }  // namespace
+
+// Dynamic LDS, from address 0 (the kernel has no other LDS): 64 zero bytes,
+// the scan scratch, then per-CDP tables and the slabs.  At the end the
+// waves' keys reuse the space from byte 64.
+template <int NW>
+struct LdsLayout {
+  static constexpr int kScan = 64;
+  static constexpr int kTr = kScan + ((4 * NW + 15) & ~15);
+  // First slab byte for a CDP of S traces: TraceRec[S], then s_pre[S+1],
+  // s_lo[S], s_row[S].
+  static __host__ __device__ constexpr int slab(int S) { return kTr + 16 * S + (((3 * S + 1) * 4 + 15) & ~15); }
+};
 
 template <int W, int R, int CC, int STORE, int NW>
 __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
-  constexpr int NT = 64 * NW;                        // threads per block
-  constexpr int PB = STORE == SB_STORE_F32 ? 8 : 4;  // bytes per pair
-  constexpr int ZP = W;                              // zero pairs (a window at 0 reads pairs 0..W-1)
-  constexpr int RP = NW * CC;                        // parameters per round (CC per wave)
+  using L = LdsLayout<NW>;
+  constexpr int NT = 64 * NW;
+  constexpr int RP = NW * CC;  // parameters per pass of the block's waves
+  constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
   extern __shared__ __align__(16) unsigned char dyn[];
-  __shared__ unsigned long long s_best[NW][64 * R];
-  __shared__ int s_wsum[NW], s_wmax[NW];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar trace indices
   const WorkItem wi = a.work[blockIdx.x];
   if (wi.cdp < 0) return;
   const int cdp = wi.cdp;
@@ -129,113 +107,75 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   if (S <= 0) return;
   const SlotF* __restrict__ sl = a.slots + k0;
 
-  // Dynamic LDS: s_pre[S+1] (exclusive prefix of the round's slab lengths),
-  // s_off[S] (s_pre[k] - first sample of slab k), s_first[S+2] (first trace
-  // of each chunk), s_h[S] (each trace's h), then the pairs.
-  int* s_pre = (int*)dyn;
-  int* s_off = s_pre + (S + 1);
-  int* s_first = s_off + S;
-  float* s_h = (float*)(s_first + S + 2);
-  const int hdr = ((4 * S + 3) * 4 + 15) & ~15;
-  unsigned char* pairs = dyn + hdr;
-  const int capP = (a.lds_bytes - hdr) / PB - ZP;  // pairs available to one chunk
+  int* s_wsum = (int*)(dyn + L::kScan);
+  TraceRec* s_tr = (TraceRec*)(dyn + L::kTr);
+  int* s_pre = (int*)(dyn + L::kTr + 16 * S);
+  int* s_lo = s_pre + (S + 1);
+  int* s_row = s_lo + S;
+  const int smp = L::slab(S);
+  const int capQ = (a.lds_bytes - smp) >> 4;  // quads the slabs can hold
 
-  for (int i = tid; i < ZP * PB / 4; i += NT) ((uint32_t*)pairs)[i] = 0u;
-  for (int k = tid; k < S; k += NT) s_h[k] = sl[k].h;  // read back below by the same thread
+  if (tid < 16) ((uint32_t*)dyn)[tid] = 0u;
+  for (int k = tid; k < S; k += NT) {  // read back below by the same thread
+    const SlotF s = sl[k];
+    TraceRec t;
+    t.h = s.h;
+    t.base = 0u;
+    t.scale = s.scale;
+    t.scale2 = s.scale2;
+    s_tr[k] = t;
+    s_row[k] = s.row;
+  }
 
   const int nwin = max(a.ns - a.w, 0);
   const int t0min = wi.tile * 64 * R;
-  const int t0max = min(t0min + 64 * R - 1, a.ns - 1);
+  const int t0max = t0min + 64 * R - 1;
   const float sqlo = __fmul_rn((float)t0min, (float)t0min), sqhi = __fmul_rn((float)t0max, (float)t0max);
-
   float T0sq[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const float T0 = (float)(wi.tile * 64 * R + r * 64 + lane);
+    const float T0 = (float)(t0min + r * 64 + lane);
     T0sq[r] = __fmul_rn(T0, T0);
   }
+  // k_search_fast's prefix path: traces before the last run ungated, the last
+  // one gated by the t^2 bound, validity from the second-to-last trace.
+  const bool prefix = t0min > a.tau;
+  const float h2 = S >= 2 ? sl[S - 2].h : 0.f;
+  // Global-memory window addressing, as k_search_fast's: FP32 rows from tau
+  // samples before their start, 16-bit rows from tau rounded up to even.
+  const int tau_even = (a.tau + 1) & ~1, even_adj = tau_even - a.tau;
+  const unsigned adj2 = 2u * (unsigned)even_adj, adj16 = 16u * (unsigned)even_adj;
   unsigned long long best[R];
 #pragma unroll
   for (int r = 0; r < R; r++) best[r] = 0ull;
 
-  for (int q0 = wi.p0; q0 < wi.p1; q0 += RP) {
-    const int q1 = min(q0 + RP, wi.p1);
-    // ---- 1. slab of every trace for this round: window starts
-    // e = floor(sqrt(T0^2 + c h)) - tau are monotone in T0 and c >= 0, so the
-    // block's extreme t0 and the round's extreme c bound them; widened by 2
-    // for the approximate square root and clipped to valid starts.
-    const float cA = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)q0));
-    const float cB = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)(q1 - 1)));
-    const float cmin = fminf(cA, cB), cmax = fmaxf(cA, cB);
-    int carry = 0, lmax = 0;
-    for (int kb = 0; kb < S; kb += NT) {
-      const int k = kb + tid;
-      int lo = 0, len = 0;
-      if (k < S) {
-        const float h = s_h[k];
-        const int elo = (int)__builtin_amdgcn_sqrtf(fmaf(cmin, h, sqlo)) - a.tau - 2;
-        const int ehi = (int)__builtin_amdgcn_sqrtf(fmaf(cmax, h, sqhi)) - a.tau + 2;
-        lo = max(elo, 0);
-        const int hi = min(ehi, nwin - 1);
-        len = hi >= lo ? hi - lo + W : 0;
-      }
-      int x = len, m = len;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-        m = max(m, __shfl_xor(m, d));
-      }
-      if (lane == 63) s_wsum[wave] = x;
-      if (lane == 0) s_wmax[wave] = m;
-      __syncthreads();
-      int off = carry, tot = 0, mx = 0;
-#pragma unroll
-      for (int q = 0; q < NW; q++) {
-        off += q < wave ? s_wsum[q] : 0;
-        tot += s_wsum[q];
-        mx = max(mx, s_wmax[q]);
-      }
-      if (k < S) {
-        s_pre[k] = off + x - len;
-        s_off[k] = off + x - len - lo;
-      }
-      carry += tot;
-      lmax = max(lmax, mx);
-      __syncthreads();
-    }
-    // ---- 2. chunks: chunk j holds the traces whose slab starts in
-    // [j*H, (j+1)*H); with H = capP - lmax a chunk never exceeds capP, and
-    // lmax <= capP / 2 keeps H >= lmax, so at most S + 1 chunks (s_first's
-    // size).  The host only launches shapes whose slabs are that narrow.
-    const int H = capP - lmax;
-    if (2 * lmax > capP) {
-      if (tid == 0) atomicOr(a.status, 1u);
-      return;
-    }
-    const int nch = carry > 0 ? (carry + H - 1) / H : 1;
-    if (tid == 0) {
-      s_pre[S] = carry;
-      s_first[0] = 0;
-      s_first[nch] = S;
-    }
-    __syncthreads();
-    for (int k = 1 + tid; k <= S; k += NT) {
-      const int jlo = s_pre[k - 1] / H, jhi = min(s_pre[k] / H, nch - 1);
-      for (int j = jlo + 1; j <= jhi; j++) s_first[j] = k;
-    }
-
-    // ---- 3. this wave's CC parameters of the round, over all chunks
-    const int p = q0 + wave * CC;
-    float pc[CC];
-    bool act[CC];
+  // ---- one wave's group of CC parameters [p, p + CC) (those below qe)
+  float pc[CC];
+  bool act[CC];
+  bool head_ok[R][CC];
+  bool wlive = true;
+  float num[R][CC][W], den[R][CC];
+  int cnt[R][CC];
+  auto begin = [&](int p, int qe) {
 #pragma unroll
     for (int ci = 0; ci < CC; ci++) {
-      act[ci] = p + ci < q1;
-      pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)(act[ci] ? p + ci : q0)));
+      act[ci] = p + ci < qe;
+      pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)(act[ci] ? p + ci : p)));
     }
-    float num[R][CC][W], den[R][CC];
-    int cnt[R][CC];
+    wlive = true;
+    if (prefix) {
+      bool live = false;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) {
+          head_ok[r][ci] = S < 2 || fmaf(pc[ci], h2, T0sq[r]) < a.hi2;
+          live = live || (head_ok[r][ci] && act[ci]);
+        }
+      // no result possible for any lane and parameter of the group: the
+      // wave skips it (same keys as k_search_fast's skipped wave)
+      wlive = __builtin_amdgcn_ballot_w64(live) != 0ull;
+    }
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -245,94 +185,156 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
 #pragma unroll
         for (int j = 0; j < W; j++) num[r][ci][j] = 0.f;
       }
-    __syncthreads();  // s_first, and the previous round's last chunk is consumed
+  };
 
-    for (int j = 0; j < nch; j++) {
-      const int ks = s_first[j], ke = s_first[j + 1];
-      if (ks >= ke) continue;  // uniform
-      const int pb0 = s_pre[ks] - ZP;  // pair index of trace k: s_pre[k] - pb0
-      // stage: one wave per trace, lanes over pairs, two loads in flight
-      for (int k = ks + wave; k < ke; k += NW) {
-        const int base = s_pre[k] - pb0, len = s_pre[k + 1] - s_pre[k], lo = s_pre[k] - s_off[k];
-        const int64_t row = sl[k].row;
-        if constexpr (STORE == SB_STORE_F32) {
-          const float* __restrict__ src = (const float*)a.samples + row * a.stride + lo;
-          for (int i = lane; i < len; i += 128) {
-            const bool two = i + 64 < len;
-            f2v u, v;
-            u.x = src[i];
-            u.y = src[i + 1];
-            if (two) {
-              v.x = src[i + 64];
-              v.y = src[i + 65];
-            }
-            *(f2v*)(pairs + (size_t)(base + i) * 8) = u;
-            if (two) *(f2v*)(pairs + (size_t)(base + i + 64) * 8) = v;
-          }
-        } else {
-          const uint16_t* __restrict__ src = (const uint16_t*)a.samples + row * a.stride + lo;
-          for (int i = lane; i < len; i += 128) {
-            const bool two = i + 64 < len;
-            const uint32_t u = (uint32_t)src[i] | ((uint32_t)src[i + 1] << 16);
-            uint32_t v = 0;
-            if (two) v = (uint32_t)src[i + 64] | ((uint32_t)src[i + 65] << 16);
-            *(uint32_t*)(pairs + (size_t)(base + i) * 4) = u;
-            if (two) *(uint32_t*)(pairs + (size_t)(base + i + 64) * 4) = v;
-          }
-        }
-      }
-      __syncthreads();
-
-      // Trace loop, software-pipelined by one trace: the windows of trace
-      // k+1 are read from LDS while trace k accumulates.
-      RawWin<W, STORE> wa[R][CC], wb[R][CC];
-      float xa[R][CC], xb[R][CC];
-      bool oa[R][CC], ob[R][CC];
-      auto issue = [&](int k, RawWin<W, STORE>(&w)[R][CC], float(&xx)[R][CC], bool(&okk)[R][CC]) {
-        const float h = s_h[k];
-        const int sbk = s_off[k] - pb0;  // pair index of window start e is sbk + e
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-          for (int ci = 0; ci < CC; ci++) {
-            const float tt = fmaf(pc[ci], h, T0sq[r]);
-            const float ts = __builtin_amdgcn_sqrtf(tt);  // tt >= 0: the host checks c0, c1 >= 0
-            const int e = (int)ts - a.tau;
-            const bool ok = (unsigned)e < (unsigned)nwin && tt < a.hi2;  // the global kernel's rule
-            okk[r][ci] = ok;
-            xx[r][ci] = __builtin_amdgcn_fractf(ts);
-            read_win<W, STORE>(pairs, ok ? (unsigned)(sbk + e) : 0u, w[r][ci]);
-          }
-        }
-      };
-      auto consume = [&](int k, const RawWin<W, STORE>(&w)[R][CC], const float(&xx)[R][CC],
-                         const bool(&okk)[R][CC]) {
-        float scale = 1.f, scale2 = 1.f;
-        if constexpr (STORE != SB_STORE_F32) {
-          const SlotF s = sl[k];
-          scale = s.scale;
-          scale2 = s.scale2;
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-          for (int ci = 0; ci < CC; ci++) {
-            cnt[r][ci] += okk[r][ci] ? 1 : 0;
-            acc_win<W, STORE>(w[r][ci], xx[r][ci], scale, scale2, num[r][ci], den[r][ci]);
-          }
-        }
-      };
-      issue(ks, wa, xa, oa);
-      for (int k = ks; k < ke; k += 2) {
-        issue(k + 1 < ke ? k + 1 : k, wb, xb, ob);
-        consume(k, wa, xa, oa);
-        if (k + 1 >= ke) break;
-        issue(k + 2 < ke ? k + 2 : k + 1, wa, xa, oa);
-        consume(k + 1, wb, xb, ob);
-      }
-      __syncthreads();  // chunk consumed before the next one is staged
+  // Reads the windows of trace `t` (LDS: `t.base`; global memory: `row`) for
+  // every (row, parameter).  GATE 0: window and t^2 test (general path);
+  // 1: none; 2: the t^2 bound (last trace).  Both sources give the same
+  // values: the stored samples converted exactly to FP32, zeros for gated
+  // windows.
+  auto issue = [&](const TraceRec& t, int row, Ev<W>(&ev)[R][CC], auto gate_tag, auto lds_tag) {
+    constexpr int GATE = decltype(gate_tag)::value;
+    constexpr bool FROM_LDS = decltype(lds_tag)::value;
+    __amdgpu_buffer_rsrc_t rs;
+    if constexpr (!FROM_LDS) {
+      const int lead = STORE == SB_STORE_F32 ? 4 * a.tau : 2 * tau_even;
+      rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ - lead), (short)0,
+          (int)(a.stride * ESZ) + lead, kRsrcWord3);
     }
-
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        const float tt = fmaf(pc[ci], t.h, T0sq[r]);
+        const float ts = __builtin_amdgcn_sqrtf(tt);  // tt >= 0: the host checks c0, c1 >= 0
+        const int it = (int)ts;
+        bool ok = true;
+        if constexpr (GATE == 0) ok = (unsigned)(it - a.tau) < (unsigned)nwin && tt < a.hi2;
+        if constexpr (GATE == 2) ok = tt < a.hi2;
+        ev[r][ci].ok = ok;
+        ev[r][ci].x = __builtin_amdgcn_fractf(ts);
+        if constexpr (FROM_LDS) {
+          unsigned addr = lshl2_add(it, t.base);
+          if constexpr (GATE != 1) addr = ok ? addr : 0u;
+#pragma unroll
+          for (int j = 0; j <= W; j++) ev[r][ci].v[j] = lds_at<float>(addr + 4u * j);
+        } else {
+          unsigned off, shf = 0u;
+          if constexpr (STORE == SB_STORE_F32) {
+            off = (unsigned)it * 4u;
+          } else {
+            off = lshl1_add(it, adj2) & ~3u;
+            shf = lshl4_add(it, adj16);
+          }
+          if constexpr (GATE != 1) off = ok ? off : kOob;
+          const Win<W + 1> wv = load_win<W + 1, STORE>(rs, off, shf);
+#pragma unroll
+          for (int j = 0; j <= W; j++) ev[r][ci].v[j] = wv.v[j];
+        }
+      }
+    }
+  };
+  // k_search_fast's window arithmetic, operation for operation.
+  auto consume = [&](const TraceRec& t, const Ev<W>(&ev)[R][CC], auto gate_tag) {
+    constexpr int GATE = decltype(gate_tag)::value;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int ci = 0; ci < CC; ci++) {
+        const Ev<W>& e = ev[r][ci];
+        if constexpr (GATE != 1) cnt[r][ci] += e.ok ? 1 : 0;
+        const float x = e.x;
+        if constexpr (STORE == SB_STORE_F32) {
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
+            num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
+            den[r][ci] = fmaf(v, v, den[r][ci]);
+          }
+        } else if constexpr (STORE == SB_STORE_F16) {
+          float q = 0.f;
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const float v = fmaf(-x, e.v[j], fmaf(x, e.v[j + 1], e.v[j]));
+            num[r][ci][j] = fmaf(t.scale, v, num[r][ci][j]);
+            q = fmaf(v, v, q);
+          }
+          den[r][ci] = fmaf(t.scale2, q, den[r][ci]);
+        } else {
+          float q = 0.f;
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
+            num[r][ci][j] = fmaf(t.scale, v, num[r][ci][j]);
+            q = fmaf(v, v, q);
+          }
+          den[r][ci] = fmaf(t.scale2, q, den[r][ci]);
+        }
+      }
+    }
+  };
+  auto tr = [&](int k) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 q = lds_at<u4>((unsigned)(L::kTr + 16 * k));
+    TraceRec t;
+    t.h = __uint_as_float(q.x);
+    t.base = q.y;
+    t.scale = __uint_as_float(q.z);
+    t.scale2 = __uint_as_float(q.w);
+    return t;
+  };
+  // Traces [kb, ke): each trace's windows are read and accumulated in turn
+  // (the waves of the CU hide the latency), the next trace's record read
+  // ahead (LDS table, or the slot table by scalar loads).
+  auto run = [&](int kb, int ke, auto gate_tag, auto lds_tag) {
+    if constexpr (decltype(lds_tag)::value) {
+      TraceRec tn = tr(kb);
+      for (int k = kb; k < ke; k++) {
+        const TraceRec t = tn;
+        tn = tr(min(k + 1, ke - 1));
+        Ev<W> A[R][CC];
+        issue(t, 0, A, gate_tag, lds_tag);
+        consume(t, A, gate_tag);
+      }
+    } else {
+      SlotF nx = sl[kb];
+      for (int k = kb; k < ke; k++) {
+        const SlotF s = nx;
+        nx = sl[min(k + 1, ke - 1)];
+        TraceRec t;
+        t.h = s.h;
+        t.base = 0u;
+        t.scale = s.scale;
+        t.scale2 = s.scale2;
+        Ev<W> A[R][CC];
+        issue(t, s.row, A, gate_tag, lds_tag);
+        consume(t, A, gate_tag);
+      }
+    }
+  };
+  // All traces of the group.
+  auto accumulate = [&](auto lds_tag) {
+    if (prefix) {
+      if (S > 1) run(0, S - 1, std::integral_constant<int, 1>{}, lds_tag);
+      run(S - 1, S, std::integral_constant<int, 2>{}, lds_tag);
+    } else {
+      run(0, S, std::integral_constant<int, 0>{}, lds_tag);
+    }
+  };
+  // The group's results into the running keys.
+  auto finish = [&](int p) {
+    if (prefix) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) cnt[r][ci] += S - 1;
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int ci = 0; ci < CC; ci++) head_ok[r][ci] = (S - cnt[r][ci]) < 2;
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
 #pragma unroll
         for (int jj = 0; jj < W; jj++) acsq = fmaf(num[r][ci][jj], num[r][ci][jj], acsq);
         const float m = (float)cnt[r][ci];
-        const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && (S - cnt[r][ci]) < 2;
+        const bool valid = act[ci] && den[r][ci] > 1e-13f && cnt[r][ci] > 0 && head_ok[r][ci];
         const float sem = __fmul_rn(acsq, __builtin_amdgcn_rcpf(__fmul_rn(den[r][ci], m)));
         if (valid && sem > 0.f) {
           const unsigned long long key =
@@ -350,21 +352,141 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         }
       }
     }
+  };
+
+  // ---- the work item's parameter range [p0, p1): c in [cmin, cmax]
+  const float cA = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)wi.p0));
+  const float cB = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)(wi.p1 - 1)));
+  const float cmin = fminf(cA, cB), cmax = fmaxf(cA, cB);
+  // No lane of the block has a result when the smallest c gives an
+  // out-of-range second-to-last t^2 at the first t0 (the same float values
+  // as that lane's); block-uniform.
+  const bool dead = prefix && S >= 2 && !(fmaf(cmin, h2, sqlo) < a.hi2);
+  bool staged = false;
+  if (!dead) {
+    // ---- each trace's range of window starts: e = floor(sqrt(T0^2 + c h))
+    // - tau is monotone in T0 and in c >= 0, so (first t0, cmin) and (last
+    // t0, cmax) bound every lane's, widened by 2 for the approximate square
+    // root.  As 16-byte quads of samples aligned as in the row (a staged quad
+    // is one 16-byte load and one aligned LDS store), prefix-summed.
+    int carry = 0;
+    for (int kb = 0; kb < S; kb += NT) {
+      const int k = kb + tid;
+      int lo = 0, len = 0;
+      if (k < S) {
+        const float h = s_tr[k].h;
+        const int elo = max((int)__builtin_amdgcn_sqrtf(fmaf(cmin, h, sqlo)) - a.tau - 2, 0);
+        const int ehi = (int)__builtin_amdgcn_sqrtf(fmaf(cmax, h, sqhi)) - a.tau + 2;
+        lo = elo >> 2;
+        len = ehi >= elo ? ((ehi + W) >> 2) - lo + 1 : 0;  // quads of samples elo .. ehi + W
+      }
+      int x = len;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63) s_wsum[wave] = x;
+      __syncthreads();
+      int off = carry, tot = 0;
+#pragma unroll
+      for (int q = 0; q < NW; q++) {
+        off += q < wave ? s_wsum[q] : 0;
+        tot += s_wsum[q];
+      }
+      if (k < S) {
+        s_pre[k] = off + x - len;
+        s_lo[k] = lo;
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    if (tid == 0) s_pre[S] = carry;
+    // Stage when every range fits (block-uniform); otherwise the waves read
+    // their windows from global memory as k_search_fast does.
+    staged = carry <= capQ;
+    if (staged) {
+      __syncthreads();  // s_pre[S]
+      // A wave takes G consecutive traces at a time, lanes over their
+      // quads, every load of the group in flight before the stores; past the
+      // row the loads read zeros.  Sets each trace's window address base.
+      constexpr int G = 8;
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      typedef unsigned u2 __attribute__((ext_vector_type(2)));
+      for (int kg = wave * G; kg < S; kg += NW * G) {
+        __amdgpu_buffer_rsrc_t rs[G];
+        int qlo[G], nq[G];
+        unsigned dq[G];
+        int nqm = 0;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          const int k = min(kg + g, S - 1);
+          const int qb = __builtin_amdgcn_readfirstlane(s_pre[k]);
+          qlo[g] = __builtin_amdgcn_readfirstlane(s_lo[k]);
+          nq[g] = kg + g < S ? __builtin_amdgcn_readfirstlane(s_pre[k + 1]) - qb : 0;
+          nqm = max(nqm, nq[g]);
+          dq[g] = (unsigned)(smp + 16 * qb);
+          if (lane == 0 && kg + g < S) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
+          const int row = __builtin_amdgcn_readfirstlane(s_row[k]);
+          rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
+                                                    (short)0, (int)(a.stride * ESZ), kRsrcWord3);
+        }
+        for (int u = lane; u < nqm; u += 64) {
+          f4 v[G];
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            const unsigned o = (unsigned)(qlo[g] + u) * (4 * ESZ);
+            if constexpr (STORE == SB_STORE_F32) {
+              v[g] = __builtin_amdgcn_raw_buffer_load_b128(rs[g], o, 0, 0);
+            } else {
+              const u2 d = __builtin_amdgcn_raw_buffer_load_b64(rs[g], o, 0, 0);
+              // element by element: a bit_cast of a vector element reads element 0
+              const unsigned d0 = d.x, d1 = d.y;
+              if constexpr (STORE == SB_STORE_F16) {
+                typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+                const h2v lo2 = __builtin_bit_cast(h2v, d0);
+                const h2v hi2 = __builtin_bit_cast(h2v, d1);
+                v[g] = f4{(float)lo2.x, (float)lo2.y, (float)hi2.x, (float)hi2.y};
+              } else {
+                v[g] = f4{__uint_as_float(d0 << 16), __uint_as_float(d0 & 0xffff0000u), __uint_as_float(d1 << 16),
+                          __uint_as_float(d1 & 0xffff0000u)};
+              }
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < G; g++)
+            if (u < nq[g]) *(__attribute__((address_space(3))) f4*)(size_t)(dq[g] + 16u * u) = v[g];
+        }
+      }
+      __syncthreads();
+    }
+    // ---- every wave takes groups of CC parameters in turn
+    for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
+      begin(p, wi.p1);
+      if (!wlive) continue;
+      if (staged)
+        accumulate(std::true_type{});
+      else
+        accumulate(std::false_type{});
+      finish(p);
+    }
   }
 
+  __syncthreads();  // every wave is done with the tables and slabs
+  unsigned long long* s_best = (unsigned long long*)(dyn + 64);
 #pragma unroll
-  for (int r = 0; r < R; r++) s_best[wave][r * 64 + lane] = best[r];
+  for (int r = 0; r < R; r++) s_best[wave * 64 * R + r * 64 + lane] = best[r];
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      unsigned long long m = s_best[0][r * 64 + lane];
+      unsigned long long m = s_best[r * 64 + lane];
 #pragma unroll
       for (int q = 1; q < NW; q++) {
-        const unsigned long long v = s_best[q][r * 64 + lane];
+        const unsigned long long v = s_best[q * 64 * R + r * 64 + lane];
         m = v > m ? v : m;
       }
-      const int t0 = wi.tile * 64 * R + r * 64 + lane;
+      const int t0 = t0min + r * 64 + lane;
       if (t0 < a.ns && m) atomicMax(&a.keys[(int64_t)(cdp - a.cdp_base) * a.ns + t0], m);
     }
   }
@@ -394,13 +516,24 @@ static hipError_t lds_w(const SearchArgs& a, int store, int nblocks, hipStream_t
   }
 }
 
+int lds_slab_offset(const FastVariant& v, int S) {
+  switch (v.waves) {
+    case 4: return LdsLayout<4>::slab(S);
+    case 8: return LdsLayout<8>::slab(S);
+    case 16: return LdsLayout<16>::slab(S);
+    default: return -1;
+  }
+}
+
 hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& v, int nblocks, hipStream_t st) {
   if (nblocks <= 0) return hipSuccess;
   if (a.method != SB_METHOD_CMP || a.lds_bytes <= 0 || a.lds_bytes > kLdsMaxBytes) return hipErrorInvalidValue;
+  if (a.lds_bytes < 64 + v.waves * 64 * v.R * 8) return hipErrorInvalidValue;  // the keys at the end
   if (v.R == 1 && v.CC == 4 && v.waves == 8) return lds_w<1, 4, 8>(a, store, nblocks, st);
+  if (v.R == 1 && v.CC == 4 && v.waves == 16) return lds_w<1, 4, 16>(a, store, nblocks, st);
   if (v.R == 2 && v.CC == 2 && v.waves == 8) return lds_w<2, 2, 8>(a, store, nblocks, st);
-  if (v.R == 1 && v.CC == 2 && v.waves == 8) return lds_w<1, 2, 8>(a, store, nblocks, st);
-  if (v.R == 1 && v.CC == 4 && v.waves == 4) return lds_w<1, 4, 4>(a, store, nblocks, st);
+  if (v.R == 2 && v.CC == 2 && v.waves == 16) return lds_w<2, 2, 16>(a, store, nblocks, st);
+  if (v.R == 2 && v.CC == 2 && v.waves == 4) return lds_w<2, 2, 4>(a, store, nblocks, st);
   return hipErrorInvalidValue;
 }
 

@@ -639,21 +639,17 @@ int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
   return pchunk;
 }
 
-// Whether k_search_lds can stage every CDP of [cb, ce): the slab table of
-// the largest CDP must fit, and the widest slab of one trace in one round --
-// at most 64R t0 + sqrt(dc * h_max) samples of moveout (sqrt(a + x) - sqrt(a)
-// <= sqrt(x)) plus rounding margins and the window -- must fit half of what
-// is left, which the kernel requires.
+// Whether k_search_lds can run every CDP of [cb, ce): the per-trace tables
+// of the largest CDP must fit its LDS with room for slabs.  (A workgroup
+// whose window ranges do not fit the rest reads its windows from global
+// memory, with the same results.)
 bool lds_fits(const sb_gather* g, const SearchArgs& a, const FastVariant& v, int cb, int ce) {
+  (void)a;
   int smax = 0;
   for (int i = cb; i < ce; i++) smax = std::max(smax, g->cdp_off[i + 1] - g->cdp_off[i]);
   if (smax > kLdsMaxTraces) return false;
-  const int pb = g->store == SB_STORE_F32 ? 8 : 4;
-  const int hdr = ((4 * smax + 3) * 4 + 15) & ~15;
-  const int64_t capP = ((int64_t)v.lds_kb * 1024 - hdr) / pb - a.w;
-  const double dc = std::fabs((double)a.finc_c) * (v.waves * v.CC - 1) * 1.001 + 1e-6;
-  const double lmax = 64.0 * v.R + std::ceil(std::sqrt(dc * std::max(0.0, (double)g->cmp_hmax))) + 8 + a.w;
-  return capP > 0 && 2 * lmax <= (double)capP;
+  const int smp = lds_slab_offset(v, smax);
+  return smp >= 0 && (int64_t)smp + 4096 <= (int64_t)v.lds_kb * 1024;
 }
 
 int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int mode, double params_per_tuple,
@@ -692,9 +688,11 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
       }
       const int tile = 64 * v.R;
       const int ntiles = (g->ns + tile - 1) / tile;
-      const int pchunk =
-          ctx->pchunk_override > 0 ? ctx->pchunk_override
-                                   : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
+      // The LDS-staged search stages one range per workgroup: by default one
+      // group of CC parameters per wave.
+      const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override
+                         : lds                    ? v.waves * v.CC
+                                                  : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
       const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
       if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
     }

@@ -10,6 +10,7 @@
 
 #include <cfloat>
 #include <string>
+#include <vector>
 
 #include "launch.hpp"
 #include "runtime.hpp"
@@ -61,12 +62,15 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
   double best = DBL_MAX;
   int best_v = save_v, best_p = save_p;
   for (int v = 0; v < kNumFastVariants && !rc; v++) {
-    const int cc = kFastVariants[v].CC;
+    const FastVariant& fv = kFastVariants[v];
+    const bool lds = fv.lds_kb > 0;
+    if (lds && method != SB_METHOD_CMP) continue;  // CMP only
+    // Global-memory shapes: automatic split or fractions of the grid.
+    // LDS-staged shapes: one or two groups of CC parameters per wave.
     const int cands[] = {0, npar, npar / 2, npar / 4, npar / 8, npar / 16};
-    for (int pc : cands) {
-      if (pc != 0 && pc < 4 * cc) continue;
-      // LDS-staged variants (CMP only) size their own parameter chunks.
-      if (kFastVariants[v].lds_kb > 0 && (method != SB_METHOD_CMP || pc != 0)) continue;
+    const int lds_cands[] = {0, 2 * fv.waves * fv.CC};
+    for (int pc : lds ? std::vector<int>(lds_cands, lds_cands + 2) : std::vector<int>(cands, cands + 6)) {
+      if (pc != 0 && pc < 4 * fv.CC) continue;
       ctx->fast_variant = v;
       ctx->pchunk_override = pc;
       double tsum = 0;

@@ -2,9 +2,10 @@
 
 Both kernels do the same arithmetic on the same stored bits in the same trace
 order, so every output must be bit-identical -- for every store, window
-length, launch shape, parameter-grid direction and on gathers whose slabs
-need several LDS chunks per parameter round (wide offset range, long
-traces)."""
+length, launch shape, parameter-grid direction, with one staging shared by
+several parameter groups per wave, and on gathers whose window ranges do not
+fit the LDS (wide offset range, long traces), where workgroups read their
+windows from global memory."""
 import numpy as np
 import pytest
 
@@ -13,8 +14,8 @@ pytestmark = pytest.mark.gpu
 KEYS = ("index", "c", "coher", "stack")
 
 
-def _search(sb, ctx, g, prm, variant):
-    ctx.tune_set(variant, 0)
+def _search(sb, ctx, g, prm, variant, pchunk=0):
+    ctx.tune_set(variant, pchunk)
     out = sb.search(ctx, g, prm)
     return out, ctx.timing()
 
@@ -37,20 +38,22 @@ def test_lds_equals_global(sb, gpu_ctx, store, tau, dt_us):
             ref, t0 = _search(sb, gpu_ctx, g, prm, 0)
             assert t0["lds_staged"] == 0
             for v in _lds_variants(sb, gpu_ctx):
-                out, t = _search(sb, gpu_ctx, g, prm, v)
-                assert t["lds_staged"] == 1, (v, t)
-                for k in KEYS:
-                    assert np.array_equal(out[k], ref[k]), (v, c0, k)
+                for pc in (0, 57):
+                    out, t = _search(sb, gpu_ctx, g, prm, v, pc)
+                    assert t["lds_staged"] == 1, (v, t)
+                    for k in KEYS:
+                        assert np.array_equal(out[k], ref[k]), (v, pc, c0, k)
     finally:
         gpu_ctx.tune_set(0, 0)
         g.close()
 
 
 @pytest.mark.parametrize("store", ["f32", "f16"])
-def test_lds_many_chunks(sb, gpu_ctx, store):
-    """Half-offsets to 2.5 km, a wide c range and w = 7: every parameter
-    round's slabs span several LDS chunks; the results must not change."""
-    print("case: lds many chunks %s" % store, flush=True)
+def test_lds_ranges_beyond_lds(sb, gpu_ctx, store):
+    """Half-offsets to 2.5 km, a wide c range and w = 7: with 200 parameters
+    per workgroup the window ranges do not fit the LDS and those workgroups
+    read their windows from global memory; the results must not change."""
+    print("case: lds ranges beyond lds %s" % store, flush=True)
     st = sb.STORE_F32 if store == "f32" else sb.STORE_F16
     off, geom, s = sb.synth(0, 3, (90, 120), 2001, 2000, 5, h_max=2500.0, nevents=5)
     g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=st)
@@ -58,10 +61,11 @@ def test_lds_many_chunks(sb, gpu_ctx, store):
         prm = sb.cmp_params(1.0e-7, 1.5e-6, 200, 0.006)
         ref, _ = _search(sb, gpu_ctx, g, prm, 0)
         for v in _lds_variants(sb, gpu_ctx):
-            out, t = _search(sb, gpu_ctx, g, prm, v)
-            assert t["lds_staged"] == 1, (v, t)
-            for k in KEYS:
-                assert np.array_equal(out[k], ref[k]), (v, k)
+            for pc in (0, 200):  # 200: the wide staging does not fit, narrow rounds run
+                out, t = _search(sb, gpu_ctx, g, prm, v, pc)
+                assert t["lds_staged"] == 1, (v, t)
+                for k in KEYS:
+                    assert np.array_equal(out[k], ref[k]), (v, pc, k)
     finally:
         gpu_ctx.tune_set(0, 0)
         g.close()

@@ -36,8 +36,9 @@ def test_launch_shapes_bit_identical(sb, gpu_ctx, store, tau):
         outs = []
         nv = gpu_ctx.tune_get()[2]
         for v in range(nv):
-            gpu_ctx.tune_set(v, 0)
-            outs.append((v, sb.search(gpu_ctx, g, prm)))
+            for pc in (0, 40):  # 40 of 72 parameters per workgroup: one LDS staging for several groups
+                gpu_ctx.tune_set(v, pc)
+                outs.append(((v, pc), sb.search(gpu_ctx, g, prm)))
         ref = outs[0][1]
         assert (ref["index"] >= 0).any() and (ref["index"] < 0).any()
         for v, o in outs[1:]:
