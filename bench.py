@@ -81,9 +81,11 @@ def panel_latency(sb, ctx, cdp_off, geom, samples, n=100):
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
-# Global-memory launch shapes of the search (rows per lane R, parameters per
-# pass CC), in the library's variant numbering; variants >= 4 are LDS-staged.
+# Launch shapes of the search in the library's variant numbering: rows per
+# lane R, parameters per pass CC, waves per workgroup; variants >= 4 are
+# LDS-staged (k_search_lds: one group of CC parameters per wave and pass).
 GLOBAL_SHAPES = {0: (2, 4), 1: (1, 4), 2: (4, 2), 3: (2, 2)}
+LDS_SHAPES = {4: (1, 4, 8), 5: (1, 4, 8), 6: (1, 4, 8), 7: (1, 4, 8)}
 
 
 def skipped_fraction(cdp_off, geom, variant, pchunk, nblocks):
@@ -93,9 +95,13 @@ def skipped_fraction(cdp_off, geom, variant, pchunk, nblocks):
     reaches (ns - tau - 1)^2 -- in a tile past tau.  Those (t0, c) have no
     result in any arithmetic.  Estimated in float64 from the geometry and the
     launch shape (the kernel decides in FP32)."""
-    if variant not in GLOBAL_SHAPES:
+    if variant in GLOBAL_SHAPES:
+        R, CC = GLOBAL_SHAPES[variant]
+        NW = 0
+    elif variant in LDS_SHAPES:
+        R, CC, NW = LDS_SHAPES[variant]
+    else:
         return 0.0
-    R, CC = GLOBAL_SHAPES[variant]
     ns, nc, dt = CFG["ns"], CFG["nc"], CFG["dt_us"] * 1e-6
     tau = int(np.float32(CFG["tau"]) * np.float32(1.0 / np.float32(dt)))
     hi = ns - tau - 1
@@ -113,10 +119,16 @@ def skipped_fraction(cdp_off, geom, variant, pchunk, nblocks):
     ncdp = len(cdp_off) - 1
     if pchunk <= 0:  # automatic split: as many chunks as the launch used
         pchunk = max(1, -(-nc // max(1, nblocks // (ncdp * ntiles))))
-    # parameter groups of one block's chunk, per wave, CC at a time
+    # parameter groups of one block's chunk, per wave, CC at a time: a
+    # contiguous run per wave (global-memory kernel) or CC-groups dealt to
+    # the waves in turn (LDS-staged kernel)
     groups = []
     for p0 in range(0, nc, pchunk):
         p1 = min(nc, p0 + pchunk)
+        if NW:
+            for p in range(p0, p1, CC):
+                groups.append((p, min(p + CC, p1)))
+            continue
         per = -(-(p1 - p0) // (4 * CC)) * CC
         for w in range(4):
             pw0, pw1 = p0 + w * per, min(p0 + (w + 1) * per, p1)
@@ -507,8 +519,10 @@ def main():
     f16 = results.get("f16")
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
     skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
+    lds = bool(f32["lds_staged"])
+    kernel = "k_search_lds" if lds else "k_search_fast"
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_k_search_fast_cmp_2000cdp.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_cmp_2000cdp.json" % kernel)
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -536,15 +550,18 @@ def main():
         "panel_latency_ms": {"p50": p50, "p99": p99, "what": "1-CDP CMP panel, upload+search+download, sequential"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VECTOR_TFLOPS, "traffic": traffic,
-                     "kernel": "k_search_fast (CMP, W=3)", "kernel_ms": f32["search_ms"],
+                     "kernel": kernel + " (CMP, W=3)", "kernel_ms": f32["search_ms"],
                      "flop_per_eval": FLOP_PER_TUPLE_CMP,
                      "skipped_frac": skip, "achieved_evaluated": achieved * (1.0 - skip),
                      "frac_evaluated": achieved * (1.0 - skip) / PEAK_FP32_VECTOR_TFLOPS,
                      "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s "
                              "(packed FP32; 78.6 TFLOP/s unpacked); achieved counts the reference's evaluations, "
                              "skipped_frac of which have no result (two invalid traces) and are never run -- "
-                             "achieved_evaluated counts only those run; the kernel is co-bound by the vector-memory "
-                             "address path (16-byte window loads) and VALU issue"},
+                             "achieved_evaluated counts only those run; " + (
+                                 "windows come from LDS (two ds_read2_b32 per evaluation, the trace ranges of a "
+                                 "workgroup staged once): bound by VALU issue with the staging latency partly hidden"
+                                 if lds else "the kernel is co-bound by the vector-memory address path (16-byte "
+                                 "window loads) and VALU issue")},
     }
     if not args.no_cpu:
         print("bench: CPU baseline", flush=True)
