@@ -422,8 +422,10 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
 
 def launch_modes(sb, ctx, g, prm, steps=3):
     """Search ms of the same step under the other launch modes (same
-    results): the persistent per-XCD grid, the LDS-staged kernel (best of its
-    shapes) and a replayed HIP graph of the tuned shape (whole step)."""
+    results): the global-memory kernel per work item and as a persistent
+    per-XCD grid (R=1, CC=4, 16 parameters per workgroup), the LDS-staged
+    kernel (best of its shapes) and a replayed HIP graph of the tuned shape
+    (whole step)."""
     v0, pc0, nv = ctx.tune_get()
     r = sb.DeviceResult(ctx, g.ncdp, g.ns)
     out = {}
@@ -435,6 +437,8 @@ def launch_modes(sb, ctx, g, prm, steps=3):
                 sb.run(ctx, g, prm, r)
                 ms.append(ctx.timing()["search_ms"])
             return float(np.median(ms))
+        ctx.tune_set(1, 16)
+        out["global_search_ms"] = time_it()
         ctx.set_option(sb.OPT_PERSISTENT, 1)
         out["persistent_search_ms"] = time_it()
         ctx.set_option(sb.OPT_PERSISTENT, 0)

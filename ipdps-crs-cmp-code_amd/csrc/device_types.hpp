@@ -79,7 +79,6 @@ struct SearchArgs {
   const WorkItem* work;
   int nwork;
   unsigned int* queue;       // persistent-kernel counters (8, one per XCD) or null
-  unsigned int* status;      // bit 0: an LDS slab did not fit (must stay 0)
   int lds_bytes;             // dynamic LDS per block of the LDS-staged search
 };
 

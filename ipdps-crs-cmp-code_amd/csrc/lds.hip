@@ -7,19 +7,19 @@
 // window loads, two aligned ds_read2_b32 cost about half of that and leave
 // the loop bound by its VALU work.
 //
-// A block works through its parameter range in rounds of NW * CC consecutive
-// parameters (CC per wave).  Per round it bounds, for every trace, the window
-// starts that any of its lanes can reach: e = floor(sqrt(T0^2 + c h)) - tau is
-// monotone in T0 and in c >= 0, so the block's first t0 with the round's
-// smallest c and its last t0 (t0min + 64R - 1, past ns for the last tile, so
+// A workgroup first bounds, for every trace, the window starts that any of
+// its lanes can reach over its whole parameter range: e = floor(sqrt(T0^2 +
+// c h)) - tau is monotone in T0 and in c >= 0, so the first t0 with the
+// smallest c and the last t0 (t0min + 64R - 1, past ns for the last tile, so
 // that every lane's reads stay inside the staged range) with the largest c
-// bound it, widened by 2 for the approximate square root.  The traces' ranges
-// are packed into chunks that fit the LDS; each chunk is staged as FP32 (the
-// stored bits converted exactly: 16-bit stores keep their per-trace scale
-// for the arithmetic) and the trace loop runs from LDS, the windowed sums
-// staying in registers across chunks.  A window start e of trace k lives at
-// byte address base_k + 4 (e + tau) (one v_lshl_add_u32 per evaluation);
-// gated windows read the zeros at address 0.
+// bound it, widened by 2 for the approximate square root.  When all ranges
+// fit, they are staged once as FP32 in 16-byte quads aligned as in the row
+// (the stored bits converted exactly: 16-bit stores keep their per-trace
+// scale for the arithmetic), and every wave then takes groups of CC
+// parameters in turn, reading each window at byte address base_k + 4 floor(t)
+// (one v_lshl_add_u32); gated windows read the zeros at address 0.  When they
+// do not fit, the waves read their windows from global memory as
+// k_search_fast does.
 //
 // Every rounding step matches k_search_fast, in the same trace order, with
 // the same general/prefix split, last-trace gate and second-to-last-trace
@@ -298,17 +298,17 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         consume(t, A, gate_tag);
       }
     } else {
-      SlotF nx = sl[kb];
+      // records and rows from the LDS tables (uniform: scalar row addressing)
+      const unsigned rowb = (unsigned)((const unsigned char*)s_row - dyn);
+      TraceRec tn = tr(kb);
+      int rn = lds_at<int>(rowb + 4u * kb);
       for (int k = kb; k < ke; k++) {
-        const SlotF s = nx;
-        nx = sl[min(k + 1, ke - 1)];
-        TraceRec t;
-        t.h = s.h;
-        t.base = 0u;
-        t.scale = s.scale;
-        t.scale2 = s.scale2;
+        const TraceRec t = tn;
+        const int row = __builtin_amdgcn_readfirstlane(rn);
+        tn = tr(min(k + 1, ke - 1));
+        rn = lds_at<int>(rowb + 4u * min(k + 1, ke - 1));
         Ev<W> A[R][CC];
-        issue(t, s.row, A, gate_tag, lds_tag);
+        issue(t, row, A, gate_tag, lds_tag);
         consume(t, A, gate_tag);
       }
     }
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
     if (tid == 0) s_pre[S] = carry;
     // Stage when every range fits (block-uniform); otherwise the waves read
     // their windows from global memory as k_search_fast does.
-    staged = carry <= capQ;
+    staged = __builtin_amdgcn_readfirstlane(carry) <= capQ;  // uniform (carry is)
     if (staged) {
       __syncthreads();  // s_pre[S]
       // A wave takes G consecutive traces at a time, lanes over their

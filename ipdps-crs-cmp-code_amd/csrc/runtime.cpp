@@ -448,7 +448,7 @@ extern "C" int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out) 
   const size_t n = (size_t)ncdp * ns;
   int rc;
   void* p = nullptr;
-  if ((rc = ctx_alloc(ctx, n * (6 * 4 + 8) + 8, &p))) return rc;
+  if ((rc = ctx_alloc(ctx, n * (6 * 4 + 8), &p))) return rc;
   r->base = p;
   r->index = (int32_t*)p;
   r->a = (float*)((char*)p + n * 4);
@@ -457,12 +457,6 @@ extern "C" int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out) 
   r->coher = (float*)((char*)p + n * 16);
   r->stack = (float*)((char*)p + n * 20);
   r->keys = (unsigned long long*)((char*)p + n * 24);
-  r->status = (unsigned int*)((char*)p + n * 32);
-  hipError_t e = launch_fill_u64((unsigned long long*)r->status, 1, 0ull, ctx->stream);
-  if (e != hipSuccess) {
-    ctx_free(ctx, p);
-    return hip_fail(e, "result status");
-  }
   *out = r.release();
   return SB_OK;
 }
@@ -486,10 +480,7 @@ extern "C" int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_resul
            {out->c, r->c},         {out->coher, r->coher}, {out->stack, r->stack}};
   for (auto& x : m)
     if (x.h) HIPR(hipMemcpyAsync(x.h, x.d, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-  unsigned int status = 0;
-  HIPR(hipMemcpyAsync(&status, r->status, 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPR(hipStreamSynchronize(ctx->stream));
-  if (status) return fail(SB_ERR_STATE, "a search block's LDS slab overflowed; results are incomplete");
   return SB_OK;
 }
 
@@ -665,7 +656,6 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   int variant = -1;
   bool lds = false;
   FastVariant v{};
-  a.status = r->status;
   // Windows the fast kernel is not compiled for (w > 7) run in reference
   // arithmetic: slower, same parity bar.
   if (mode == SB_MODE_FAST && !fast_supported(a.w)) mode = SB_MODE_EXACT;

@@ -73,7 +73,6 @@ struct sb_result {
   int32_t* index = nullptr;
   float *a = nullptr, *b = nullptr, *c = nullptr, *coher = nullptr, *stack = nullptr;
   unsigned long long* keys = nullptr;
-  unsigned int* status = nullptr;  // set by a search that could not complete (checked on download)
 };
 
 namespace sb {
