@@ -24,6 +24,9 @@ constexpr int kLdsMaxTraces = 2048;
 int lds_slab_offset(const FastVariant& v, int S);
 extern const FastVariant kFastVariants[];
 extern const int kNumFastVariants;
+// Shape of an untuned ctx: LDS-staged, 8 waves, 52 KiB (CMP; searches it
+// cannot stage run the first global-memory shape).
+constexpr int kDefaultFastVariant = 5;
 
 bool fast_supported(int w);
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "device_types.hpp"
+#include "launch.hpp"
 #include "semblance.h"
 
 struct sb_ctx {
@@ -24,7 +25,7 @@ struct sb_ctx {
   bool timing_graph = false;  // the pending timing is a graph replay
   int num_cus = 256;
   // launch shape of the fast search (autotuned or set by sb_tune_set)
-  int fast_variant = 0;
+  int fast_variant = sb::kDefaultFastVariant;
   int pchunk_override = 0;
   std::map<std::string, std::pair<int, int>> tune_cache;
   // reusable work table / queue counters
