@@ -1,0 +1,89 @@
+"""The LDS-staged CMP search reads every window from its workgroup's staged
+sample ranges (CPU check of the bound, no GPU).
+
+k_search_lds stages, per trace of a (CDP, 64-t0 tile, parameter chunk), the
+16-byte quads that hold samples max(elo, 0) .. ehi + W, where
+
+    elo = floor(sqrt(fma(cmin, h, t0min^2))) - tau - 2
+    ehi = floor(sqrt(fma(cmax, h, (t0min + 63)^2))) - tau + 2
+
+(FP32, the square root of the hardware, accurate to 1 ulp).  A lane reads the
+window starting at e = floor(sqrt(fma(c, h, T0^2))) - tau.  In the prefix
+path (t0min > tau) every lane reads its window without a validity test --
+also lanes whose result is discarded and lanes past the end of the trace --
+so each such window must lie inside the staged quads; in the general path
+only valid windows (0 <= e < ns - w and t^2 < hi2) are read.  This test
+evaluates both sides in FP32 with the square roots pushed 1 ulp the wrong
+way (the bound's inward, the lanes' outward) on random lines, parameter
+grids and both grid directions, including the last, partial tile."""
+import numpy as np
+
+f32 = np.float32
+
+
+def _fma(a, b, c):
+    # float32 fused multiply-add through float64 (the product is exact)
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
+
+
+def _sqrt(x, direction):
+    r = np.sqrt(np.maximum(x, f32(0))).astype(np.float32)
+    return np.nextafter(r, np.float32(np.inf if direction > 0 else -np.inf)).astype(np.float32)
+
+
+def _check(rng, ns, tau, c0, c1, nc, chunk, S, hmax):
+    w = 2 * tau + 1
+    nwin = ns - w
+    hi = ns - tau - 1
+    hi2 = f32(hi * hi * (1.0 - 2.0 ** -20))
+    inc = f32((f32(c1) - f32(c0)) * f32(1.0 / nc))
+    h = np.sort(rng.uniform(0, hmax, S).astype(np.float32))
+    lanes = np.arange(64)
+    bad = 0
+    for tile in range(0, (ns + 63) // 64):
+        t0min = tile * 64
+        T0 = (t0min + lanes).astype(np.float32)
+        T0sq = (T0 * T0).astype(np.float32)
+        sqlo = f32(f32(t0min) * f32(t0min))
+        sqhi = f32(f32(t0min + 63) * f32(t0min + 63))
+        prefix = t0min > tau
+        for q0 in range(0, nc, chunk):
+            q1 = min(q0 + chunk, nc)
+            cA = f32(f32(c0) + f32(inc * f32(q0)))
+            cB = f32(f32(c0) + f32(inc * f32(q1 - 1)))
+            cmin, cmax = min(cA, cB), max(cA, cB)
+            q = np.arange(q0, q1)
+            pc = (f32(c0) + (inc * q.astype(np.float32)).astype(np.float32)).astype(np.float32)
+            # the kernel's per-trace range, square roots pushed inward
+            elo = np.maximum(_sqrt(_fma(np.full(S, cmin, np.float32), h, np.full(S, sqlo, np.float32)), +1)
+                             .astype(np.int64) - tau - 2, 0)
+            ehi = _sqrt(_fma(np.full(S, cmax, np.float32), h, np.full(S, sqhi, np.float32)), -1) \
+                .astype(np.int64) - tau + 2
+            first = (elo >> 2) * 4                       # first staged sample
+            last = ((ehi + w) >> 2) * 4 + 3              # last staged sample
+            # every lane x parameter x trace, square roots pushed outward
+            tt = _fma(pc[None, :, None], h[None, None, :], T0sq[:, None, None])
+            for d in (-1, +1):
+                e = _sqrt(tt, d).astype(np.int64) - tau
+                if prefix:
+                    read = np.ones_like(e, dtype=bool)
+                else:
+                    read = (e >= 0) & (e < nwin) & (tt < hi2)
+                out = read & ((e < first[None, None, :]) | (e + w > last[None, None, :]) |
+                              (ehi[None, None, :] < elo[None, None, :]))
+                bad += int(out.sum())
+    return bad
+
+
+def test_lds_staged_ranges_cover_every_window():
+    rng = np.random.default_rng(2024)
+    cases = [
+        # ns, tau, c0, c1, nc, chunk, S, hmax (h in samples^2 per unit of c)
+        (1501, 1, 0.198, 1.77, 256, 32, 120, 5.6e5),   # the headline line's scales
+        (733, 0, 0.198, 1.77, 57, 57, 150, 5.6e5),
+        (733, 3, 1.77, 0.198, 57, 32, 90, 5.6e5),      # decreasing grid
+        (301, 2, 0.1, 3.0, 72, 40, 60, 2.0e6),          # far offsets, short traces
+        (2001, 3, 0.1, 1.5, 200, 64, 40, 1.6e6),
+    ]
+    for case in cases:
+        assert _check(rng, *case) == 0, case
