@@ -285,7 +285,6 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
   // read one dword past the last sample.
   G->stride = ((int64_t)ns + 8 + 63) / 64 * 64;
   G->cdp_off.assign(cdp_off, cdp_off + ncdp + 1);
-  G->cmp_hmax = INFINITY;  // known once every slot exists (finish)
   const size_t esz = store == SB_STORE_F32 ? 4 : 2;
   int rc;
   // kSamplesLead bytes before row 0: the FP32 search addresses a row from
@@ -391,11 +390,6 @@ int GatherBuild::batch(int cb, int ce, hipEvent_t ev, int* ready) {
 int GatherBuild::finish(sb_gather** out) {
   HIPR(hipStreamSynchronize(cs));
   sb_gather* g = G.get();
-  // Largest scaled h: bounds the LDS slabs of the CMP search.
-  std::vector<SlotF> hs(g->ntr);
-  HIPR(hipMemcpy(hs.data(), g->d_cmp_slots, (size_t)g->ntr * sizeof(SlotF), hipMemcpyDeviceToHost));
-  g->cmp_hmax = 0.f;
-  for (int i = 0; i < g->ntr; i++) g->cmp_hmax = std::max(g->cmp_hmax, hs[i].h);
   g->bytes = 0;
   {
     std::lock_guard<std::mutex> lk(ctx->mem_mu);

@@ -63,7 +63,6 @@ struct sb_gather {
   sb::SlotF* d_crs_slots_p = nullptr;  // unscaled FP32 geometry (CRS-DE exact mode)
   int64_t ncrs_slots = 0;
   std::vector<int32_t> cdp_off, crs_slot_off;  // host copies
-  float cmp_hmax = 0.f;                        // largest CMP slot h
   size_t bytes = 0;
 };
 
