@@ -220,7 +220,8 @@ __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
 // (Kahan summation) -- used to report 16-bit-store results, whose inputs
 // already differ from the reference's, so that FP32 accumulation adds no
 // error of its own; FP32 stores keep the reference's plain sums (and bits).
-template <int MAXW = SB_MAX_W, bool KAHAN = false, class LD>
+// EXACT_W: the window is exactly MAXW samples (the per-sample guards vanish).
+template <int MAXW = SB_MAX_W, bool KAHAN = false, bool EXACT_W = false, class LD>
 __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int cid, float& sem,
                                float& stk) {
 #pragma clang fp contract(off)
@@ -228,7 +229,7 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   float tt0 = a.dt_f * (float)t0;
   tt0 = tt0 * tt0;
   const float lo = (float)a.tau, hi = (float)(a.ns - a.tau - 1);
-  const int w = a.w;
+  const int w = EXACT_W ? MAXW : a.w;
   float num[MAXW];
 #pragma unroll
   for (int j = 0; j < MAXW; j++) num[j] = 0.f;
@@ -302,7 +303,7 @@ __device__ void eval_cmp_exact(const SearchArgs& a, const LD& ld, int cdp, int t
 }
 
 // One (CDP, t0, par) of CRS exactly as CRS/OpenMP/src/main.cpp:210-265: double.
-template <int MAXW = SB_MAX_W, class LD>
+template <int MAXW = SB_MAX_W, bool EXACT_W = false, class LD>
 __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t0, int p, double& sem,
                                double& stk) {
 #pragma clang fp contract(off)
@@ -312,7 +313,7 @@ __device__ void eval_crs_exact(const SearchArgs& a, const LD& ld, int cdp, int t
   const double pc = a.dc0 + a.dinc_c * (double)(p % a.nc);
   const double tt0 = a.dt_d * (double)t0;
   const double lo = (double)a.tau, hi = (double)(a.ns - a.tau - 1);
-  const int w = a.w;
+  const int w = EXACT_W ? MAXW : a.w;
   double num[MAXW];
 #pragma unroll
   for (int j = 0; j < MAXW; j++) num[j] = 0.0;
@@ -454,7 +455,7 @@ constexpr int kFinalizeMaxW = 7;
 
 // Decodes the packed best key of every (CDP, t0) and re-evaluates the winner
 // in the reference's precision and order; writes the output arrays.
-template <int MAXW, class LD>
+template <int MAXW, bool EXACT_W, class LD>
 __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int64_t n = (int64_t)a.nout * a.ns;
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,19 +465,19 @@ __global__ __launch_bounds__(256) void k_finalize(SearchArgs a, LD ld) {
   const int p = key ? (int)(~(uint32_t)(key & 0xffffffffu)) : -1;
   if (a.method == SB_METHOD_CMP) {
     float sem = 0.f, stk = 0.f;
-    if (p >= 0) eval_cmp_exact<MAXW, LD::kQuantized>(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_cmp_exact<MAXW, LD::kQuantized, EXACT_W>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.f)
       write_cmp(a, o, p, sem, stk);
     else
       write_cmp(a, o, -1, 0.f, 0.f);
   } else {
     double sem = 0.0, stk = 0.0;
-    if (p >= 0) eval_crs_exact<MAXW>(a, ld, cdp, t0, p, sem, stk);
+    if (p >= 0) eval_crs_exact<MAXW, EXACT_W>(a, ld, cdp, t0, p, sem, stk);
     if (p >= 0 && sem > 0.0) {
       write_crs(a, o, p, sem, stk);
     } else {
       double s0;
-      eval_crs_exact<MAXW>(a, ld, cdp, 0, 0, s0, stk);
+      eval_crs_exact<MAXW, EXACT_W>(a, ld, cdp, 0, 0, s0, stk);
       write_crs(a, o, 0, 0.0, stk);
     }
   }
@@ -1090,10 +1091,18 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 template <class LD>
 static hipError_t finalize_with(const SearchArgs& a, const LD& ld, hipStream_t st) {
   const int64_t n = (int64_t)a.nout * a.ns;
-  if (a.w <= kFinalizeMaxW)
-    hipLaunchKernelGGL((k_finalize<kFinalizeMaxW, LD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
-  else
-    hipLaunchKernelGGL((k_finalize<SB_MAX_W, LD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ld);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  switch (a.w) {  // the common windows compiled exactly
+    case 1: hipLaunchKernelGGL((k_finalize<1, true, LD>), grid, dim3(256), 0, st, a, ld); break;
+    case 3: hipLaunchKernelGGL((k_finalize<3, true, LD>), grid, dim3(256), 0, st, a, ld); break;
+    case 5: hipLaunchKernelGGL((k_finalize<5, true, LD>), grid, dim3(256), 0, st, a, ld); break;
+    case 7: hipLaunchKernelGGL((k_finalize<7, true, LD>), grid, dim3(256), 0, st, a, ld); break;
+    default:
+      if (a.w <= kFinalizeMaxW)
+        hipLaunchKernelGGL((k_finalize<kFinalizeMaxW, false, LD>), grid, dim3(256), 0, st, a, ld);
+      else
+        hipLaunchKernelGGL((k_finalize<SB_MAX_W, false, LD>), grid, dim3(256), 0, st, a, ld);
+  }
   return SB_CHECK_LAUNCH();
 }
 

@@ -481,35 +481,40 @@ extern "C" int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_resul
 // --------------------------------------------------------------- searches
 namespace sb {
 
-// Work table: CDPs in descending cost, dealt round-robin over the 8 XCDs
-// (consecutive workgroups land on consecutive XCDs), so every block of a
-// CDP shares one L2 and heavy CDPs start first on every XCD.
-// Deals per-CDP item lists to the 8 XCDs (heaviest CDP first, each to the
-// least-loaded XCD), interleaves them so consecutive workgroups land on
-// consecutive XCDs, and uploads the table.
-static int deal_work(sb_ctx* ctx, std::vector<std::vector<WorkItem>>& per_cdp, const std::vector<int64_t>& cost,
-                     int* nblocks, const std::string& key) {
-  const int ncdp = (int)per_cdp.size();
-  std::vector<int> order(ncdp);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+// Work table of CDPs [cb, ce): every CDP contributes ntiles x nchunks items
+// (tile-major, parameter chunks inner).  CDPs are dealt to the 8 XCDs by
+// decreasing cost onto the least-loaded one, and each XCD's items go to the
+// block indices of that XCD (b % 8: consecutive workgroups land on
+// consecutive XCDs), so every block of a CDP shares one L2 and heavy CDPs
+// start first on every XCD.  Written straight into the pinned staging
+// buffer and copied to the device on the ctx stream.
+int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
+               int* nblocks, const std::string& key) {
+  if (!key.empty() && key == ctx->work_key) {
+    *nblocks = ctx->work_n;
+    return SB_OK;
+  }
+  ctx->work_key.clear();
   constexpr int kXcd = 8;
-  std::vector<std::vector<WorkItem>> lists(kXcd);
+  const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
+  const int64_t per = (int64_t)ntiles * nchunks;  // items per CDP
+  const int ncdp = ce - cb;
+  std::vector<int> order(ncdp);
+  std::iota(order.begin(), order.end(), cb);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<std::vector<int>> lists(kXcd);
   std::vector<int64_t> load(kXcd, 0);
-  for (int q = 0; q < ncdp; q++) {
-    const int i = order[q];
+  for (int i : order) {
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[x] += cost[i];
-    lists[x].insert(lists[x].end(), per_cdp[i].begin(), per_cdp[i].end());
+    lists[x].push_back(i);
   }
-  size_t longest = 0;
-  for (auto& l : lists) longest = std::max(longest, l.size());
-  std::vector<WorkItem> items(longest * kXcd, WorkItem{-1, 0, 0, 0});
+  // Item j of XCD x sits at index j * 8 + x; the table ends after the last
+  // real item (the shorter lists' tails in between are no-ops).
+  int64_t n = 0;
   for (int x = 0; x < kXcd; x++)
-    for (size_t j = 0; j < lists[x].size(); j++) items[j * kXcd + x] = lists[x][j];
-  // Trim trailing no-ops.
-  while (!items.empty() && items.back().cdp < 0) items.pop_back();
-  const size_t bytes = std::max<size_t>(items.size(), 1) * sizeof(WorkItem);
+    if (!lists[x].empty()) n = std::max<int64_t>(n, ((int64_t)lists[x].size() * per - 1) * kXcd + x + 1);
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(WorkItem);
   if (bytes > ctx->work_cap) {
     ctx_free(ctx, ctx->work_buf);
     ctx->work_buf = nullptr;
@@ -518,47 +523,35 @@ static int deal_work(sb_ctx* ctx, std::vector<std::vector<WorkItem>>& per_cdp, c
     if (rc) return rc;
     ctx->work_cap = bytes * 2;
   }
-  if (!items.empty()) {
-    // Staged through a pinned host buffer: a true asynchronous copy, ordered
-    // on the stream.  The buffer is rewritten only once the previous table's
-    // copy has completed (work_ev).
-    const size_t n = items.size() * sizeof(WorkItem);
+  if (n > 0) {
+    // The staging buffer is rewritten only once the previous table's copy
+    // has completed (work_host_ev).
+    const size_t nb = (size_t)n * sizeof(WorkItem);
     if (ctx->work_host_ev) HIPR(hipEventSynchronize(ctx->work_host_ev));
-    if (n > ctx->work_host_cap) {
+    if (nb > ctx->work_host_cap) {
       if (ctx->work_host) (void)hipHostFree(ctx->work_host);
       ctx->work_host = nullptr;
       ctx->work_host_cap = 0;
-      HIPR(hipHostMalloc(&ctx->work_host, n * 2, hipHostMallocDefault));
-      ctx->work_host_cap = n * 2;
+      HIPR(hipHostMalloc(&ctx->work_host, nb * 2, hipHostMallocDefault));
+      ctx->work_host_cap = nb * 2;
     }
-    std::memcpy(ctx->work_host, items.data(), n);
-    HIPR(hipMemcpyAsync(ctx->work_buf, ctx->work_host, n, hipMemcpyHostToDevice, ctx->stream));
+    WorkItem* items = (WorkItem*)ctx->work_host;
+    for (int x = 0; x < kXcd; x++) {
+      int64_t j = 0;
+      for (int i : lists[x])
+        for (int t = 0; t < ntiles; t++)
+          for (int k = 0; k < nchunks; k++, j++)
+            items[j * kXcd + x] = WorkItem{i, t, k * pchunk, std::min(npar, (k + 1) * pchunk)};
+      for (; j * kXcd + x < n; j++) items[j * kXcd + x] = WorkItem{-1, 0, 0, 0};
+    }
+    HIPR(hipMemcpyAsync(ctx->work_buf, ctx->work_host, nb, hipMemcpyHostToDevice, ctx->stream));
     if (!ctx->work_host_ev) HIPR(hipEventCreateWithFlags(&ctx->work_host_ev, hipEventDisableTiming));
     HIPR(hipEventRecord(ctx->work_host_ev, ctx->stream));
   }
-  *nblocks = (int)items.size();
+  *nblocks = (int)n;
   ctx->work_key = key;
   ctx->work_n = *nblocks;
   return SB_OK;
-}
-
-int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
-               int* nblocks, const std::string& key) {
-  if (!key.empty() && key == ctx->work_key) {
-    *nblocks = ctx->work_n;
-    return SB_OK;
-  }
-  ctx->work_key.clear();
-  const int nchunks = std::max(1, (npar + pchunk - 1) / pchunk);
-  std::vector<std::vector<WorkItem>> per_cdp(ce - cb);
-  std::vector<int64_t> c(ce - cb);
-  for (int i = cb; i < ce; i++) {
-    c[i - cb] = cost[i];
-    for (int t = 0; t < ntiles; t++)
-      for (int k = 0; k < nchunks; k++)
-        per_cdp[i - cb].push_back(WorkItem{i, t, k * pchunk, std::min(npar, (k + 1) * pchunk)});
-  }
-  return deal_work(ctx, per_cdp, c, nblocks, key);
 }
 
 }  // namespace sb
