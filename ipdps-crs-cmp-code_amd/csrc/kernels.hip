@@ -403,8 +403,9 @@ __device__ void write_crs(const SearchArgs& a, int64_t o, int idx, double sem, d
 // One thread per (CDP, t0) scanning every parameter in order with the
 // reference's strict '>' arg-max (CMP/OpenMP/src/main.cpp:207-222,
 // CRS/OpenMP/src/main.cpp:270-285).  Work items carry (cdp, 64-t0 tile).
-// MAXW: the largest window this instantiation runs (7 or SB_MAX_W).
-template <int MAXW, class LD>
+// MAXW: the largest window this instantiation runs (7 or SB_MAX_W), or with
+// EXACT_W the window itself.
+template <int MAXW, bool EXACT_W, class LD>
 __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
   const WorkItem wi = a.work[blockIdx.x];
   if (wi.cdp < 0) return;
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
     int max_c = -1;
     for (int c = 0; c < a.nc; c++) {
       float sem, stk;
-      eval_cmp_exact<MAXW>(a, ld, wi.cdp, t0, c, sem, stk);
+      eval_cmp_exact<MAXW, false, EXACT_W>(a, ld, wi.cdp, t0, c, sem, stk);
       if (sem > max_sem) {
         max_sem = sem;
         max_c = c;
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
     int max_p = -1;
     for (int p = 0; p < a.npar; p++) {
       double sem, stk;
-      eval_crs_exact<MAXW>(a, ld, wi.cdp, t0, p, sem, stk);
+      eval_crs_exact<MAXW, EXACT_W>(a, ld, wi.cdp, t0, p, sem, stk);
       if (sem > max_sem) {
         max_sem = sem;
         max_p = p;
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(64) void k_search_exact(SearchArgs a, LD ld) {
       // No parameter beat 0: the reference keeps the absolute index 0, i.e.
       // (t0 = 0, par 0) -- stack = stt[0] of this CDP (quirk Q7).
       double sem0;
-      eval_crs_exact<MAXW>(a, ld, wi.cdp, 0, 0, sem0, best_stk);
+      eval_crs_exact<MAXW, EXACT_W>(a, ld, wi.cdp, 0, 0, sem0, best_stk);
       max_p = 0;
     }
     write_crs(a, o, max_p, max_sem, best_stk);
@@ -1108,10 +1109,17 @@ static hipError_t finalize_with(const SearchArgs& a, const LD& ld, hipStream_t s
 
 template <class LD>
 static hipError_t exact_with(const SearchArgs& a, const LD& ld, int nblocks, hipStream_t st) {
-  if (a.w <= 7)
-    hipLaunchKernelGGL((k_search_exact<7, LD>), dim3(nblocks), dim3(64), 0, st, a, ld);
-  else
-    hipLaunchKernelGGL((k_search_exact<SB_MAX_W, LD>), dim3(nblocks), dim3(64), 0, st, a, ld);
+  switch (a.w) {  // the common windows compiled exactly
+    case 1: hipLaunchKernelGGL((k_search_exact<1, true, LD>), dim3(nblocks), dim3(64), 0, st, a, ld); break;
+    case 3: hipLaunchKernelGGL((k_search_exact<3, true, LD>), dim3(nblocks), dim3(64), 0, st, a, ld); break;
+    case 5: hipLaunchKernelGGL((k_search_exact<5, true, LD>), dim3(nblocks), dim3(64), 0, st, a, ld); break;
+    case 7: hipLaunchKernelGGL((k_search_exact<7, true, LD>), dim3(nblocks), dim3(64), 0, st, a, ld); break;
+    default:
+      if (a.w <= 7)
+        hipLaunchKernelGGL((k_search_exact<7, false, LD>), dim3(nblocks), dim3(64), 0, st, a, ld);
+      else
+        hipLaunchKernelGGL((k_search_exact<SB_MAX_W, false, LD>), dim3(nblocks), dim3(64), 0, st, a, ld);
+  }
   return SB_CHECK_LAUNCH();
 }
 
