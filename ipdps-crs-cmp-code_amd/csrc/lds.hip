@@ -65,7 +65,7 @@ __device__ __forceinline__ T lds_at(unsigned addr) {
 // One trace of the CDP as the trace loop reads it (one 16-byte LDS read).
 struct TraceRec {
   float h;
-  unsigned base;  // LDS byte address of window start -tau in the current chunk
+  unsigned base;  // the window of traveltime t starts at LDS byte base + 4 floor(t)
   float scale, scale2;
 };
 
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   const int capQ = (a.lds_bytes - smp) >> 4;  // quads the slabs can hold
 
   if (tid < 16) ((uint32_t*)dyn)[tid] = 0u;
-  for (int k = tid; k < S; k += NT) {  // read back below by the same thread
+  for (int k = tid; k < S; k += NT) {  // h read back below by the same thread
     const SlotF s = sl[k];
     TraceRec t;
     t.h = s.h;
