@@ -133,8 +133,6 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
   const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
   const int nrec = (int)(a.stride * ESZ);
-  const uint32_t row_bytes = (uint32_t)(a.stride * ESZ);
-  const char* __restrict__ base = (const char*)a.samples;
   float num[W], den = 0.f;
   int cnt = 0;
 #pragma unroll
@@ -149,8 +147,8 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   constexpr bool HALF = STORE == SB_STORE_F16;
   typedef typename std::conditional<HALF, WinH<W + 1>, Win<W + 1>>::type WinT;
   auto issue = [&](const SlotF& s, WinT& sv, float& x, bool& ok) {
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)(base + (uint64_t)(uint32_t)s.row * row_bytes), (short)0, nrec, kRsrcWord3);
+    // the slot carries its row's address: no per-trace address arithmetic
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)s.addr, (short)0, nrec, kRsrcWord3);
     // (T0 + a m)^2 + b m^2 + c h expanded: three FMAs per trace
     const float tt = fmaf(K2, s.m2, fmaf(K1, s.m, fmaf(pc, s.h, T0sq)));
     // t^2 < 0 is invalid (the reference's t = -1): clamp to keep the
@@ -198,25 +196,42 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
     }
   };
   // Slots are read two traces ahead of their use, so the scalar loads are in
-  // flight while a trace accumulates (indices clamped to the last trace; a
-  // repeat is loaded, never summed).
+  // flight while a trace accumulates.  The main loop runs while two more
+  // slots exist, so it reads them by a walking pointer with no clamping; the
+  // last one to three traces follow.  Traces are summed in order.
   if (k0 < k1) {
     WinT wa, wb;
     float xa, xb;
     bool oa, ob;
-    SlotF sa = a.slots[k0];
-    SlotF sb = a.slots[k0 + 1 < k1 ? k0 + 1 : k1 - 1];
+    const SlotF* __restrict__ sp = a.slots + k0;
+    SlotF sa = sp[0];
     issue(sa, wa, xa, oa);
-    for (int k = k0; k < k1; k += 2) {
-      const SlotF sc = a.slots[k + 2 < k1 ? k + 2 : k1 - 1];
+    if (k1 - k0 >= 2) {
+      SlotF sb = sp[1];
+      int k = k0;
+      for (; k + 3 < k1; k += 2, sp += 2) {  // traces k, k+1 summed; k+2, k+3 exist
+        const SlotF sc = sp[2];
+        issue(sb, wb, xb, ob);
+        consume(sa, wa, xa, oa);
+        const SlotF sd = sp[3];
+        issue(sc, wa, xa, oa);
+        consume(sb, wb, xb, ob);
+        sa = sc;
+        sb = sd;
+      }
+      // traces k (issued), k + 1 and, when k + 2 < k1, k + 2
       issue(sb, wb, xb, ob);
       consume(sa, wa, xa, oa);
-      if (k + 1 >= k1) break;
-      const SlotF sd = a.slots[k + 3 < k1 ? k + 3 : k1 - 1];
-      issue(sc, wa, xa, oa);
-      consume(sb, wb, xb, ob);
-      sa = sc;
-      sb = sd;
+      if (k + 2 < k1) {
+        const SlotF sc = sp[2];
+        issue(sc, wa, xa, oa);
+        consume(sb, wb, xb, ob);
+        consume(sc, wa, xa, oa);
+      } else {
+        consume(sb, wb, xb, ob);
+      }
+    } else {
+      consume(sa, wa, xa, oa);
     }
   }
   float acsq = 0.f, lin = 0.f;

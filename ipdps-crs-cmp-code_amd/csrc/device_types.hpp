@@ -11,12 +11,13 @@ namespace sb {
 // geometry scaled to samples: m * idt, m^2 * idt^2, h * idt^2 where m is the
 // midpoint distance to the CDP's reference point (m) and h = half-offset^2/1e6.
 // Computed in FP64 on the device, stored FP32, with the row's amplitude
-// scale (1 for FP32 storage).  32 bytes: one scalar load.
+// scale (1 for FP32 storage) and, in CRS fast slots, the byte address of the
+// row's first sample.  32 bytes: one scalar load.
 struct SlotF {
   int32_t row;
   float m, m2, h;
   float scale, scale2;
-  int32_t pad0, pad1;
+  uint64_t addr;
 };
 
 // Reference-arithmetic slot for SB_MODE_EXACT CRS (CRS/OpenMP/src/main.cpp:189-198):

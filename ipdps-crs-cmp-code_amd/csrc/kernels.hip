@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_cmp_slots(int cdp0, const int32_t* __re
       s.h = h;
       s.scale = scale ? scale[r] : 1.f;
       s.scale2 = s.scale * s.scale;
-      s.pad0 = s.pad1 = 0;
+      s.addr = 0;
       slots[r0 + rank] = s;
     }
   }
@@ -116,7 +116,8 @@ __global__ void k_crs_slots(int cdp0, const int32_t* __restrict__ cdp_off, const
                             const int32_t* __restrict__ nbr, const int32_t* __restrict__ slot_off,
                             const double* __restrict__ m0x, const double* __restrict__ m0y,
                             const double* __restrict__ h0, const float* __restrict__ scale, double idt,
-                            SlotF* __restrict__ sf, SlotD* __restrict__ sd, SlotF* __restrict__ sp) {
+                            const char* samples, uint64_t row_bytes, SlotF* __restrict__ sf,
+                            SlotD* __restrict__ sd, SlotF* __restrict__ sp) {
 #pragma clang fp contract(off)
   const int i = cdp0 + blockIdx.x;
   const int ref = cdp_off[i];
@@ -147,7 +148,7 @@ __global__ void k_crs_slots(int cdp0, const int32_t* __restrict__ cdp_off, const
       f.h = (float)(h * idt * idt);
       f.scale = scale ? scale[r] : 1.f;
       f.scale2 = f.scale * f.scale;
-      f.pad0 = f.pad1 = 0;
+      f.addr = (uint64_t)(samples + (uint64_t)r * row_bytes);
       sf[base + (r - r0)] = f;
       f.m = (float)m;
       f.m2 = (float)m2;
@@ -1064,10 +1065,11 @@ hipError_t launch_cmp_slots(int cdp0, int ncdp, const int32_t* cdp_off, const do
 
 hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
                             const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
-                            const float* scale, double idt, SlotF* sf, SlotD* sd, SlotF* sp, hipStream_t st) {
+                            const float* scale, double idt, const void* samples, uint64_t row_bytes, SlotF* sf,
+                            SlotD* sd, SlotF* sp, hipStream_t st) {
   if (ncdp <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_crs_slots, dim3(ncdp), dim3(256), 0, st, cdp0, cdp_off, nbr_off, nbr, slot_off, m0x, m0y, h0,
-                     scale, idt, sf, sd, sp);
+                     scale, idt, (const char*)samples, row_bytes, sf, sd, sp);
   return SB_CHECK_LAUNCH();
 }
 

@@ -377,8 +377,8 @@ int GatherBuild::batch(int cb, int ce, hipEvent_t ev, int* ready) {
     int r = crs_ready;
     while (r < G->ncdp && nbr_reach[r] < ce) r++;
     HIPR(launch_crs_slots(crs_ready, r - crs_ready, G->d_cdp_off, (const int32_t*)dnoff, (const int32_t*)dnbr,
-                          G->d_crs_slot_off, G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_crs_slots,
-                          G->d_crs_slots_d, G->d_crs_slots_p, cs));
+                          G->d_crs_slot_off, G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_samples,
+                          (uint64_t)G->stride * esz, G->d_crs_slots, G->d_crs_slots_d, G->d_crs_slots_p, cs));
     crs_ready = r;
     done = r;
   }
