@@ -62,11 +62,18 @@ __device__ __forceinline__ T lds_at(unsigned addr) {
   return *(const __attribute__((address_space(3))) T*)(size_t)addr;
 }
 
-// One trace of the CDP as the trace loop reads it (one 16-byte LDS read).
+// One trace of the CDP as the trace loop uses it: h and the 16-bit stores'
+// scales from the slot table (scalar loads), the window base from LDS.
 struct TraceRec {
   float h;
   unsigned base;  // the window of traveltime t starts at LDS byte base + 4 floor(t)
   float scale, scale2;
+};
+
+// A trace's entry in the LDS table (h for the range scan, then the base).
+struct TraceLds {
+  float h;
+  unsigned base;
 };
 
 // One evaluation in flight: its window and interpolation fraction.
@@ -85,9 +92,9 @@ template <int NW>
 struct LdsLayout {
   static constexpr int kScan = 64;
   static constexpr int kTr = kScan + ((4 * NW + 15) & ~15);
-  // First slab byte for a CDP of S traces: TraceRec[S], then s_pre[S+1],
-  // s_lo[S], s_row[S].
-  static __host__ __device__ constexpr int slab(int S) { return kTr + 16 * S + (((3 * S + 1) * 4 + 15) & ~15); }
+  // First slab byte for a CDP of S traces: TraceLds[S], then s_pre[S+1],
+  // s_lo[S].
+  static __host__ __device__ constexpr int slab(int S) { return kTr + ((8 * S + 4 * (2 * S + 1) + 15) & ~15); }
 };
 
 template <int W, int R, int CC, int STORE, int NW>
@@ -108,24 +115,14 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   const SlotF* __restrict__ sl = a.slots + k0;
 
   int* s_wsum = (int*)(dyn + L::kScan);
-  TraceRec* s_tr = (TraceRec*)(dyn + L::kTr);
-  int* s_pre = (int*)(dyn + L::kTr + 16 * S);
+  TraceLds* s_tr = (TraceLds*)(dyn + L::kTr);
+  int* s_pre = (int*)(dyn + L::kTr + 8 * S);
   int* s_lo = s_pre + (S + 1);
-  int* s_row = s_lo + S;
   const int smp = L::slab(S);
   const int capQ = (a.lds_bytes - smp) >> 4;  // quads the slabs can hold
 
   if (tid < 16) ((uint32_t*)dyn)[tid] = 0u;
-  for (int k = tid; k < S; k += NT) {  // h read back below by the same thread
-    const SlotF s = sl[k];
-    TraceRec t;
-    t.h = s.h;
-    t.base = 0u;
-    t.scale = s.scale;
-    t.scale2 = s.scale2;
-    s_tr[k] = t;
-    s_row[k] = s.row;
-  }
+  for (int k = tid; k < S; k += NT) s_tr[k] = TraceLds{sl[k].h, 0u};  // h read back below by the same thread
 
   const int nwin = max(a.ns - a.w, 0);
   const int t0min = wi.tile * 64 * R;
@@ -274,43 +271,34 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       }
     }
   };
-  auto tr = [&](int k) {
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    const u4 q = lds_at<u4>((unsigned)(L::kTr + 16 * k));
+  // Trace k's record: h and scales by scalar loads from the slot table (the
+  // kernel only reads it), the base by a 4-byte LDS read (a 16-byte record
+  // read per trace and wave returned 1 KiB over the LDS data path the window
+  // reads use).
+  typedef const __attribute__((address_space(4))) SlotF CSlot;
+  CSlot* slc = (CSlot*)(size_t)sl;
+  auto tr = [&](int k, bool with_base) {
     TraceRec t;
-    t.h = __uint_as_float(q.x);
-    t.base = q.y;
-    t.scale = __uint_as_float(q.z);
-    t.scale2 = __uint_as_float(q.w);
+    t.h = slc[k].h;
+    t.base = with_base ? lds_at<unsigned>((unsigned)(L::kTr + 8 * k + 4)) : 0u;
+    if constexpr (STORE == SB_STORE_F32) {
+      t.scale = t.scale2 = 1.f;
+    } else {
+      t.scale = slc[k].scale;
+      t.scale2 = slc[k].scale2;
+    }
     return t;
   };
   // Traces [kb, ke): each trace's windows are read and accumulated in turn
-  // (the waves of the CU hide the latency), the next trace's record read
-  // ahead (LDS table, or the slot table by scalar loads).
+  // (the waves of the CU hide the latency); global-memory windows take the
+  // row from the slot table (scalar: uniform row addressing).
   auto run = [&](int kb, int ke, auto gate_tag, auto lds_tag) {
-    if constexpr (decltype(lds_tag)::value) {
-      TraceRec tn = tr(kb);
-      for (int k = kb; k < ke; k++) {
-        const TraceRec t = tn;
-        tn = tr(min(k + 1, ke - 1));
-        Ev<W> A[R][CC];
-        issue(t, 0, A, gate_tag, lds_tag);
-        consume(t, A, gate_tag);
-      }
-    } else {
-      // records and rows from the LDS tables (uniform: scalar row addressing)
-      const unsigned rowb = (unsigned)((const unsigned char*)s_row - dyn);
-      TraceRec tn = tr(kb);
-      int rn = lds_at<int>(rowb + 4u * kb);
-      for (int k = kb; k < ke; k++) {
-        const TraceRec t = tn;
-        const int row = __builtin_amdgcn_readfirstlane(rn);
-        tn = tr(min(k + 1, ke - 1));
-        rn = lds_at<int>(rowb + 4u * min(k + 1, ke - 1));
-        Ev<W> A[R][CC];
-        issue(t, row, A, gate_tag, lds_tag);
-        consume(t, A, gate_tag);
-      }
+    constexpr bool FROM_LDS = decltype(lds_tag)::value;
+    for (int k = kb; k < ke; k++) {
+      const TraceRec t = tr(k, FROM_LDS);
+      Ev<W> A[R][CC];
+      issue(t, FROM_LDS ? 0 : slc[k].row, A, gate_tag, lds_tag);
+      consume(t, A, gate_tag);
     }
   };
   // All traces of the group.
@@ -427,7 +415,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           nqm = max(nqm, nq[g]);
           dq[g] = (unsigned)(smp + 16 * qb);
           if (lane == 0 && kg + g < S) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
-          const int row = __builtin_amdgcn_readfirstlane(s_row[k]);
+          const int row = slc[k].row;
           rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
                                                     (short)0, (int)(a.stride * ESZ), kRsrcWord3);
         }

@@ -48,13 +48,13 @@ def test_lds_equals_global(sb, gpu_ctx, store, tau, dt_us):
         g.close()
 
 
-@pytest.mark.parametrize("store", ["f32", "f16"])
+@pytest.mark.parametrize("store", ["f32", "f16", "bf16"])
 def test_lds_ranges_beyond_lds(sb, gpu_ctx, store):
     """Half-offsets to 2.5 km, a wide c range and w = 7: with 200 parameters
     per workgroup the window ranges do not fit the LDS and those workgroups
     read their windows from global memory; the results must not change."""
     print("case: lds ranges beyond lds %s" % store, flush=True)
-    st = sb.STORE_F32 if store == "f32" else sb.STORE_F16
+    st = {"f32": sb.STORE_F32, "f16": sb.STORE_F16, "bf16": sb.STORE_BF16}[store]
     off, geom, s = sb.synth(0, 3, (90, 120), 2001, 2000, 5, h_max=2500.0, nevents=5)
     g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=st)
     try:
