@@ -403,10 +403,12 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
                 time.sleep(d)
             tickets.append((ev[1], srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5])))
         lat = {"cmp": [], "crs": [], "probe": []}
+        wait = {"cmp": [], "crs": [], "probe": []}
         for kind, tk in tickets:
             out, st = srv.wait(tk)
             if out is not None:
                 lat[kind].append(st["t_done_ms"] - st["t_submit_ms"])
+                wait[kind].append(st["t_admit_ms"] - st["t_submit_ms"])
         wall = time.perf_counter() - start
         stats = srv.stats()
     res = dict(store="f32" if store == 0 else "f16", offered_rps=rate + probe_rate, duration_s=round(wall, 3),
@@ -415,8 +417,10 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
     for k, v in lat.items():
         v = np.array(v)
         if len(v):
+            w = np.array(wait[k])
             res[k] = dict(n=len(v), p50_ms=float(np.percentile(v, 50)), p99_ms=float(np.percentile(v, 99)),
-                          max_ms=float(v.max()))
+                          max_ms=float(v.max()), queue_p50_ms=float(np.percentile(w, 50)),
+                          queue_p99_ms=float(np.percentile(w, 99)))
     return res
 
 

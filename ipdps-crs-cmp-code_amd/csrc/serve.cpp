@@ -295,18 +295,27 @@ void sb_server::worker(int slot) {
     std::shared_ptr<Req> head;
     cv_work.wait(lk, [&] {
       if (stop) return true;
-      // Oldest request that fits the free budget; a request alone larger
-      // than the whole budget is rejected.
+      // Slots 1..: the oldest request that fits the free budget.  Slot 0:
+      // the smallest small request that fits (shortest job first, so 1-CDP
+      // panels do not queue behind larger small ones), unless one waited a
+      // second.  A request alone larger than the whole budget is rejected.
+      const double now = now_ms(t0);
+      std::shared_ptr<Req> best;
       for (auto& q : pending) {
         if (slot == 0 && q->evals > kSmallEvals) continue;
+        const bool old = now - q->t_submit > 1000.0;
         if (q->bytes > budget || reserved + q->bytes <= budget) {
-          head = q;
-          return true;
+          if (slot != 0 || old) {
+            head = q;
+            return true;
+          }
+          if (!best || q->evals < best->evals) best = q;
+        } else if (old) {
+          break;  // do not let newer requests overtake one that waited a second
         }
-        // do not let small requests overtake one that waited a second
-        if (now_ms(t0) - q->t_submit > 1000.0) return false;
       }
-      return false;
+      head = best;
+      return head != nullptr;
     });
     if (stop && !head) return;
     if (!head) continue;
