@@ -12,14 +12,15 @@
 // c h)) - tau is monotone in T0 and in c >= 0, so the first t0 with the
 // smallest c and the last t0 (t0min + 64R - 1, past ns for the last tile, so
 // that every lane's reads stay inside the staged range) with the largest c
-// bound it, widened by 2 for the approximate square root.  When all ranges
-// fit, they are staged once as FP32 in 16-byte quads aligned as in the row
-// (the stored bits converted exactly: 16-bit stores keep their per-trace
-// scale for the arithmetic), and every wave then takes groups of CC
-// parameters in turn, reading each window at byte address base_k + 4 floor(t)
-// (one v_lshl_add_u32); gated windows read the zeros at address 0.  When they
-// do not fit, the waves read their windows from global memory as
-// k_search_fast does.
+// bound it, widened by 2 for the approximate square root.  The ranges of the
+// first K traces in slot order (all of them when they fit; the nearest
+// offsets have the shortest ranges) are staged once as FP32 in 16-byte quads
+// aligned as in the row (the stored bits converted exactly: 16-bit stores
+// keep their per-trace scale for the arithmetic), and every wave then takes
+// groups of CC parameters in turn, reading the windows of those traces at
+// byte address base_k + 4 floor(t) (one v_lshl_add_u32; gated windows read
+// the zeros at address 0) and the windows of the other traces from global
+// memory as k_search_fast does.
 //
 // Every rounding step matches k_search_fast, in the same trace order, with
 // the same general/prefix split, last-trace gate and second-to-last-trace
@@ -301,13 +302,23 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       consume(t, A, gate_tag);
     }
   };
-  // All traces of the group.
-  auto accumulate = [&](auto lds_tag) {
+  // All traces of the group, in slot order: traces [0, K) from their staged
+  // ranges, [K, S) from global memory.
+  auto accumulate = [&](int K) {
+    using G0 = std::integral_constant<int, 0>;
+    using G1 = std::integral_constant<int, 1>;
+    using G2 = std::integral_constant<int, 2>;
     if (prefix) {
-      if (S > 1) run(0, S - 1, std::integral_constant<int, 1>{}, lds_tag);
-      run(S - 1, S, std::integral_constant<int, 2>{}, lds_tag);
+      const int kp = min(K, S - 1);
+      if (kp > 0) run(0, kp, G1{}, std::true_type{});
+      if (S - 1 > kp) run(kp, S - 1, G1{}, std::false_type{});
+      if (K == S)
+        run(S - 1, S, G2{}, std::true_type{});
+      else
+        run(S - 1, S, G2{}, std::false_type{});
     } else {
-      run(0, S, std::integral_constant<int, 0>{}, lds_tag);
+      if (K > 0) run(0, K, G0{}, std::true_type{});
+      if (S > K) run(K, S, G0{}, std::false_type{});
     }
   };
   // The group's results into the running keys.
@@ -350,7 +361,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   // out-of-range second-to-last t^2 at the first t0 (the same float values
   // as that lane's); block-uniform.
   const bool dead = prefix && S >= 2 && !(fmaf(cmin, h2, sqlo) < a.hi2);
-  bool staged = false;
+  int K = 0;  // traces staged in LDS (the first K in slot order)
   if (!dead) {
     // ---- each trace's range of window starts: e = floor(sqrt(T0^2 + c h))
     // - tau is monotone in T0 and in c >= 0, so (first t0, cmin) and (last
@@ -389,32 +400,44 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       carry += tot;
       __syncthreads();
     }
-    if (tid == 0) s_pre[S] = carry;
-    // Stage when every range fits (block-uniform); otherwise the waves read
-    // their windows from global memory as k_search_fast does.
-    staged = __builtin_amdgcn_readfirstlane(carry) <= capQ;  // uniform (carry is)
-    if (staged) {
-      __syncthreads();  // s_pre[S]
+    // Stage the first K traces, as many as the slabs hold (all when every
+    // range fits; block-uniform): the nearest offsets, whose ranges are the
+    // shortest.  The other traces' windows are read from global memory as
+    // k_search_fast reads them.  s_pre is nondecreasing, so exactly one
+    // thread sees the boundary s_pre[K] <= capQ < s_pre[K + 1].
+    if (tid == 0) {
+      s_pre[S] = carry;
+      s_wsum[0] = 0;  // every read of the scan scratch is done
+    }
+    __syncthreads();  // s_pre[S], s_wsum[0]
+    K = S;
+    if (__builtin_amdgcn_readfirstlane(carry) > capQ) {  // uniform (carry is)
+      for (int k = tid; k < S; k += NT)
+        if (s_pre[k + 1] <= capQ && s_pre[k + 2] > capQ) s_wsum[0] = k + 1;  // k + 1 < S here
+      __syncthreads();
+      K = __builtin_amdgcn_readfirstlane(s_wsum[0]);
+    }
+    if (K > 0) {
       // A wave takes G consecutive traces at a time, lanes over their
       // quads, every load of the group in flight before the stores; past the
       // row the loads read zeros.  Sets each trace's window address base.
       constexpr int G = 8;
       typedef float f4 __attribute__((ext_vector_type(4)));
       typedef unsigned u2 __attribute__((ext_vector_type(2)));
-      for (int kg = wave * G; kg < S; kg += NW * G) {
+      for (int kg = wave * G; kg < K; kg += NW * G) {
         __amdgpu_buffer_rsrc_t rs[G];
         int qlo[G], nq[G];
         unsigned dq[G];
         int nqm = 0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
-          const int k = min(kg + g, S - 1);
+          const int k = min(kg + g, K - 1);
           const int qb = __builtin_amdgcn_readfirstlane(s_pre[k]);
           qlo[g] = __builtin_amdgcn_readfirstlane(s_lo[k]);
-          nq[g] = kg + g < S ? __builtin_amdgcn_readfirstlane(s_pre[k + 1]) - qb : 0;
+          nq[g] = kg + g < K ? __builtin_amdgcn_readfirstlane(s_pre[k + 1]) - qb : 0;
           nqm = max(nqm, nq[g]);
           dq[g] = (unsigned)(smp + 16 * qb);
-          if (lane == 0 && kg + g < S) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
+          if (lane == 0 && kg + g < K) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
           const int row = slc[k].row;
           rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
                                                     (short)0, (int)(a.stride * ESZ), kRsrcWord3);
@@ -448,15 +471,22 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       }
       __syncthreads();
     }
-    // ---- every wave takes groups of CC parameters in turn
-    for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
-      begin(p, wi.p1);
-      if (!wlive) continue;
-      if (staged)
-        accumulate(std::true_type{});
-      else
-        accumulate(std::false_type{});
-      finish(p);
+    // ---- every wave takes groups of CC parameters in turn (the fully
+    // staged case in a loop of its own: it holds only the LDS trace loops)
+    if (K == S) {
+      for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
+        begin(p, wi.p1);
+        if (!wlive) continue;
+        accumulate(S);
+        finish(p);
+      }
+    } else {
+      for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
+        begin(p, wi.p1);
+        if (!wlive) continue;
+        accumulate(K);
+        finish(p);
+      }
     }
   }
 

@@ -66,10 +66,10 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
     const bool lds = fv.lds_kb > 0;
     if (lds && method != SB_METHOD_CMP) continue;  // CMP only
     // Global-memory shapes: automatic split or fractions of the grid.
-    // LDS-staged shapes: one or two groups of CC parameters per wave.
+    // LDS-staged shapes: one, two or four groups of CC parameters per wave.
     const int cands[] = {0, npar, npar / 2, npar / 4, npar / 8, npar / 16};
-    const int lds_cands[] = {0, 2 * fv.waves * fv.CC};
-    for (int pc : lds ? std::vector<int>(lds_cands, lds_cands + 2) : std::vector<int>(cands, cands + 6)) {
+    const int lds_cands[] = {0, 2 * fv.waves * fv.CC, 4 * fv.waves * fv.CC};
+    for (int pc : lds ? std::vector<int>(lds_cands, lds_cands + 3) : std::vector<int>(cands, cands + 6)) {
       if (pc != 0 && pc < 4 * fv.CC) continue;
       ctx->fast_variant = v;
       ctx->pchunk_override = pc;
