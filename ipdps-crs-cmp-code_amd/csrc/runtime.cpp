@@ -617,6 +617,19 @@ int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
   return pchunk;
 }
 
+// Parameters per workgroup of the LDS-staged search when untuned: four, two
+// or one groups of CC per wave -- the most that still leave eight rounds of
+// three workgroups per CU (one staging then serves more evaluations; small
+// searches keep enough workgroups to spread over the CUs).
+int lds_default_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
+  const int64_t want = (int64_t)ctx->num_cus * 3 * 8;
+  for (int groups = 4; groups > 1; groups /= 2) {
+    const int pc = groups * unit;
+    if ((int64_t)ncdp * ntiles * ((npar + pc - 1) / pc) >= want) return pc;
+  }
+  return unit;
+}
+
 // Whether k_search_lds can run every CDP of [cb, ce): the per-trace tables
 // of the largest CDP must fit its LDS with room for slabs.  (A workgroup
 // whose window ranges do not fit the rest reads its windows from global
@@ -665,11 +678,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
       }
       const int tile = 64 * v.R;
       const int ntiles = (g->ns + tile - 1) / tile;
-      // The LDS-staged search stages one range per workgroup: by default one
-      // group of CC parameters per wave.
       const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override
-                         : lds                    ? v.waves * v.CC
-                                                  : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
+                         : lds ? lds_default_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC)
+                               : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
       const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
       if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
     }
