@@ -240,7 +240,8 @@ typedef struct {
   int grid_blocks;     /* main kernel launch size                */
   int block_threads;
   int variant;         /* launch shape that ran (sb_tune_get numbering; -1: exact mode) */
-  int lds_staged;      /* 1 if the windows came from LDS slabs   */
+  int lds_staged;      /* 1 if the LDS-staged kernel ran (windows from LDS slabs; traces
+                          whose ranges did not fit read from global memory) */
 } sb_timing;
 int sb_last_timing(const sb_ctx* ctx, sb_timing* t);
 
