@@ -566,8 +566,10 @@ def main():
                              "(packed FP32; 78.6 TFLOP/s unpacked); achieved counts the reference's evaluations, "
                              "skipped_frac of which have no result (two invalid traces) and are never run -- "
                              "achieved_evaluated counts only those run; " + (
-                                 "windows come from LDS (two ds_read2_b32 per evaluation, the trace ranges of a "
-                                 "workgroup staged once): bound by VALU issue with the staging latency partly hidden"
+                                 "windows come from LDS (two ds_read2_b32 per evaluation; a workgroup stages the "
+                                 "ranges of as many of its traces as fit, once, and reads the rest from global "
+                                 "memory): bound by VALU issue and the LDS data path, the staging latency partly "
+                                 "hidden"
                                  if lds else "the kernel is co-bound by the vector-memory address path (16-byte "
                                  "window loads) and VALU issue")},
     }
