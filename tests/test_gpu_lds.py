@@ -4,8 +4,9 @@ Both kernels do the same arithmetic on the same stored bits in the same trace
 order, so every output must be bit-identical -- for every store, window
 length, launch shape, parameter-grid direction, with one staging shared by
 several parameter groups per wave, and on gathers whose window ranges do not
-fit the LDS (wide offset range, long traces), where workgroups read their
-windows from global memory."""
+all fit the LDS (wide offset range, long traces, many parameters per
+workgroup), where workgroups stage their first traces and read the other
+traces' windows from global memory."""
 import numpy as np
 import pytest
 
@@ -62,6 +63,30 @@ def test_lds_ranges_beyond_lds(sb, gpu_ctx, store):
         ref, _ = _search(sb, gpu_ctx, g, prm, 0)
         for v in _lds_variants(sb, gpu_ctx):
             for pc in (0, 200):  # 200: the wide staging does not fit, narrow rounds run
+                out, t = _search(sb, gpu_ctx, g, prm, v, pc)
+                assert t["lds_staged"] == 1, (v, t)
+                for k in KEYS:
+                    assert np.array_equal(out[k], ref[k]), (v, pc, k)
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
+
+
+@pytest.mark.parametrize("store", ["f32", "f16"])
+def test_lds_partial_staging(sb, gpu_ctx, store):
+    """128 and 256 parameters per workgroup on the headline line's geometry
+    (ns 1501, nc 256, w 3): most workgroups stage only their first traces and
+    read the others' windows from global memory; every output must equal the
+    global-memory kernel's."""
+    print("case: lds partial staging %s" % store, flush=True)
+    st = sb.STORE_F32 if store == "f32" else sb.STORE_F16
+    off, geom, s = sb.synth(0, 5, (60, 120), 1501, 2000, 31, h_max=1500.0, h_jitter=5.0, nevents=8)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=st)
+    try:
+        prm = sb.cmp_params(1.98e-7, 1.77e-6, 256, 0.002)
+        ref, _ = _search(sb, gpu_ctx, g, prm, 0)
+        for v in _lds_variants(sb, gpu_ctx):
+            for pc in (128, 256):
                 out, t = _search(sb, gpu_ctx, g, prm, v, pc)
                 assert t["lds_staged"] == 1, (v, t)
                 for k in KEYS:
