@@ -130,7 +130,7 @@ int sb_crs_neighbors(int ncdp, const sb_trace_geom* first_geom, int apm,
 /* Keep-test of the aph aperture exactly as su_gather applies it: the
  * half-offset from su_trace::halfoffset_x/y (float), squared and summed in
  * float for CMP (CMP/OpenMP/src/su_gather.cpp:44-47) and in double for CRS
- * (CRS/OpenMP/src/su_gather.cpp:215-218), against aph*aph.  Returns 1 = keep. */
+ * (CRS/OpenMP/src/su_gather.cpp:44-47), against aph*aph.  Returns 1 = keep. */
 int sb_keep_trace(const sb_trace_geom* g, int method, int aph);
 
 /* ------------------------------------------------------- device context */

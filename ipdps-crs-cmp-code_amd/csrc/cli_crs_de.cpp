@@ -14,6 +14,17 @@ static double env_d(const char* k, double dflt) {
   return v ? std::atof(v) : dflt;
 }
 
+// 64-bit seeds: parsed as an unsigned integer (decimal, or hex with 0x), not
+// through a double, which would round seeds above 2^53.
+static uint64_t env_u64(const char* k, uint64_t dflt) {
+  const char* v = std::getenv(k);
+  if (!v || !*v) return dflt;
+  char* end = nullptr;
+  const unsigned long long x = std::strtoull(v, &end, 0);
+  if (end == v || *end) cli::log(cli::FAIL, std::string("bad ") + k + ": " + v);
+  return (uint64_t)x;
+}
+
 int main(int argc, const char** argv) {
   using namespace cli;
   touch_outputs({"crs.a.su", "crs.b.su", "crs.c.su", "crs.coher.su", "crs.stack.su"});
@@ -58,7 +69,7 @@ int main(int argc, const char** argv) {
   prm.pop = (int)env_d("SB_DE_POP", 32);
   prm.F = env_d("SB_DE_F", 0.5);
   prm.CR = env_d("SB_DE_CR", 0.9);
-  prm.seed = (uint64_t)env_d("SB_DE_SEED", 1);
+  prm.seed = env_u64("SB_DE_SEED", 1);
   prm.mode = env_mode();
 
   Session s;

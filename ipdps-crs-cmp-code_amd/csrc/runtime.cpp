@@ -51,9 +51,16 @@ static size_t round_block(size_t bytes) {
   return (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
 }
 
+// The free-event of a cached block goes back to the pool once the block is
+// reused or released (hipFree waits for the device anyway).
+static void drop_event(sb_ctx* ctx, hipEvent_t ev) {
+  if (ev) ctx->event_pool.push_back(ev);
+}
+
 static void release_cache(sb_ctx* ctx) {
   for (auto& kv : ctx->cache) {
-    (void)hipFree(kv.second);
+    (void)hipFree(kv.second.p);
+    drop_event(ctx, kv.second.freed);
     ctx->used -= kv.first;
   }
   ctx->cache.clear();
@@ -66,7 +73,14 @@ int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
   std::lock_guard<std::mutex> lk(ctx->mem_mu);
   auto it = ctx->cache.lower_bound(want);
   if (it != ctx->cache.end() && it->first <= 2 * want) {
-    *p = it->second;
+    // The new owner may write the block from any stream (or the host): wait
+    // until the work queued on the ctx stream before the free has finished.
+    if (it->second.freed) {
+      const hipError_t e = hipEventSynchronize(it->second.freed);
+      if (e != hipSuccess) return hip_fail(e, "cached block");
+      drop_event(ctx, it->second.freed);
+    }
+    *p = it->second.p;
     ctx->sizes[*p] = it->first;
     ctx->cache.erase(it);
     return SB_OK;
@@ -89,12 +103,30 @@ int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
   return SB_OK;
 }
 
+// Returns a block to the cache, stream-ordered: an event recorded on the ctx
+// stream marks the point after which no queued work of this ctx reads it.
+// (No block is freed while the stream is being captured into a graph; if
+// one were, or the event cannot be recorded, the stream is drained instead.)
 void ctx_free(sb_ctx* ctx, void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(ctx->mem_mu);
   auto it = ctx->sizes.find(p);
   if (it == ctx->sizes.end()) return;
-  ctx->cache.insert({it->second, p});
+  hipEvent_t ev = nullptr;
+  if (!ctx->capturing) {
+    if (!ctx->event_pool.empty()) {
+      ev = ctx->event_pool.back();
+      ctx->event_pool.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      ev = nullptr;
+    }
+    if (ev && hipEventRecord(ev, ctx->stream) != hipSuccess) {
+      ctx->event_pool.push_back(ev);
+      ev = nullptr;
+    }
+    if (!ev) (void)hipStreamSynchronize(ctx->stream);
+  }
+  ctx->cache.insert({it->second, sb_ctx::Cached{p, ev}});
   ctx->sizes.erase(it);
 }
 
@@ -104,7 +136,8 @@ void ctx_trim(sb_ctx* ctx, size_t keep) {
   for (auto& kv : ctx->cache) cached += kv.first;
   while (cached > keep && !ctx->cache.empty()) {
     auto it = std::prev(ctx->cache.end());  // largest first
-    (void)hipFree(it->second);
+    (void)hipFree(it->second.p);
+    drop_event(ctx, it->second.freed);
     ctx->used -= it->first;
     cached -= it->first;
     ctx->cache.erase(it);
@@ -117,6 +150,8 @@ void ctx_release_all(sb_ctx* ctx) {
   for (auto& kv : ctx->sizes) (void)hipFree(kv.first);
   ctx->sizes.clear();
   ctx->used = 0;
+  for (hipEvent_t ev : ctx->event_pool) (void)hipEventDestroy(ev);
+  ctx->event_pool.clear();
 }
 
 }  // namespace sb

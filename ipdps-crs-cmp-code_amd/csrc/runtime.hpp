@@ -18,7 +18,15 @@ struct sb_ctx {
   size_t budget = 0, used = 0;
   std::mutex mem_mu, run_mu;
   std::map<void*, size_t> sizes;           // live blocks
-  std::multimap<size_t, void*> cache;      // freed blocks kept for reuse
+  // Freed blocks kept for reuse, each with an event recorded on `stream` when
+  // it was freed: work queued before the free may still read the block, so it
+  // is handed out again only once that event has completed.
+  struct Cached {
+    void* p;
+    hipEvent_t freed;  // null: nothing queued could still use it
+  };
+  std::multimap<size_t, Cached> cache;
+  std::vector<hipEvent_t> event_pool;      // completed free-events for reuse
   hipEvent_t ev[4] = {};
   sb_timing last{};
   bool timing_pending = false;

@@ -1,11 +1,11 @@
 // SU file input/output and CDP binning with the reference's semantics.
 //
 // Reader:  su_trace::fgettr       CMP/OpenMP/src/su_trace.cpp:37-51
-// Filter:  su_gather::su_gather   CMP/OpenMP/src/su_gather.cpp:43-55 (CMP, float)
-//                                 CRS/OpenMP/src/su_gather.cpp:214-226 (CRS, double)
+// Filter:  su_gather::su_gather   CMP/OpenMP/src/su_gather.cpp:44-47 (CMP, float)
+//                                 CRS/OpenMP/src/su_gather.cpp:44-47 (CRS, double)
 // Order:   CMP std::sort by fold descending (su_gather.cpp:68, su_cdp.hpp:53-55)
 //          CRS ascending cdp key (std::map order)
-// Lists:   CRS neighbour CDPs within apm, CRS/OpenMP/src/su_gather.cpp:241-258
+// Lists:   CRS neighbour CDPs within apm, CRS/OpenMP/src/su_gather.cpp:68-87 (test :80-81)
 // Writer:  output sections, CMP/OpenMP/src/main.cpp:244-260 (only parameter
 //          sections rewrite the header, :249-251)
 #include "sb_internal.hpp"

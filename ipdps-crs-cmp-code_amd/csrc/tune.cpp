@@ -36,19 +36,37 @@ extern "C" int sb_tune_get(const sb_ctx* ctx, int* variant, int* pchunk, int* nv
 extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int reps) {
   if (!ctx || !g || !params || reps < 1) return fail(SB_ERR_INVALID, "sb_tune: bad arguments");
   if (method != SB_METHOD_CMP && method != SB_METHOD_CRS) return fail(SB_ERR_INVALID, "sb_tune: bad method");
-  int npar, w_key;
+  int npar, w_key, tau;
+  bool fast;  // whether the search runs a fast-mode kernel at all
   if (method == SB_METHOD_CMP) {
     const sb_cmp_params* p = (const sb_cmp_params*)params;
     npar = p->nc;
     w_key = (int)(p->tau * 1e6);
+    fast = p->mode == SB_MODE_FAST && p->c0 >= 0 && p->c1 >= 0;
+    // the window as sb_cmp_run computes it (float)
+    float dt = (float)g->dt_us;
+    dt = dt / 1000000.0f;
+    const float tt = (float)p->tau * (1.0f / dt);
+    tau = (int)tt > 0 ? (int)tt : 0;
   } else {
     const sb_crs_params* p = (const sb_crs_params*)params;
     npar = p->na * p->nb * p->nc;
     w_key = (int)(p->tau * 1e6);
+    fast = p->mode == SB_MODE_FAST;
+    // the window as sb_crs_run computes it (double)
+    double dt = (double)g->dt_us;
+    dt = dt / 1000000.0;
+    const double tt = p->tau * (1.0 / dt);
+    tau = (int)tt > 0 ? (int)tt : 0;
   }
+  if (npar <= 0) return fail(SB_ERR_INVALID, "sb_tune: empty parameter grid");
+  // Searches that run in reference arithmetic (SB_MODE_EXACT, windows above
+  // 7 samples, CMP grids with c < 0) do not use the tuned shape: nothing to
+  // time, and nothing cached under a key that fast searches would reuse.
+  if (!fast || !fast_supported(2 * tau + 1)) return SB_OK;
   const std::string key = std::to_string(method) + "/" + std::to_string(g->store) + "/" + std::to_string(w_key) + "/" +
                           std::to_string(g->ncdp) + "/" + std::to_string(g->ns) + "/" + std::to_string(npar) + "/" +
-                          std::to_string(g->ntr);
+                          std::to_string(g->ntr) + "/fast";
   auto hit = ctx->tune_cache.find(key);
   if (hit != ctx->tune_cache.end()) {
     ctx->fast_variant = hit->second.first;

@@ -476,106 +476,229 @@ static void de_eval(float a, float b, float c, float dt, float idt, int tau, int
   }
 }
 
+/* One (CDP, t0) population run.  hist (optional) records the trajectory as
+ * orc_de_history documents it (oracle.h). */
+typedef struct {
+  float* pop_x;   /* [(ngen+1)][pop][3] */
+  float* pop_f;   /* [(ngen+1)][pop] */
+  float* trial_x; /* [ngen][pop][3] */
+  float* trial_f; /* [ngen][pop] */
+  int32_t* picks; /* [ngen][pop][4] */
+} de_hist;
+
+typedef struct {
+  const float *lo, *hi;
+  float F;
+  uint32_t cr_thr;
+  uint64_t seed;
+  int pop, ngen;
+  float dt, idt;
+  int tau, w, ns, size;
+  const int* rows;
+  const float *m2, *m, *h, *samples;
+} de_setup;
+
+static void de_point(const de_setup* S, int cdp, int t0, float* best_x, float* best_f, float* best_s,
+                     const de_hist* hist) {
+  float x[64][3], fit[64], stk[64], tr[64][3], tf[64], ts[64];
+  const int pop = S->pop;
+  const float* lo = S->lo;
+  const float* hi = S->hi;
+  const uint64_t seed = S->seed;
+#define DE_EVAL(A, B, Cc, F_, S_)                                                                          \
+  de_eval((A), (B), (Cc), S->dt, S->idt, S->tau, S->w, S->ns, S->size, S->rows, S->m2, S->m, S->h, S->samples, \
+          t0, (F_), (S_))
+  for (int i = 0; i < pop; i++) {
+    for (int d = 0; d < 3; d++) x[i][d] = lo[d] + unit(orc_de_hash(seed, cdp, t0, 0, i, d)) * (hi[d] - lo[d]);
+    DE_EVAL(x[i][0], x[i][1], x[i][2], &fit[i], &stk[i]);
+  }
+  for (int gen = 1; gen <= S->ngen; gen++) {
+    if (hist)
+      for (int i = 0; i < pop; i++) {
+        for (int d = 0; d < 3; d++) hist->pop_x[((size_t)(gen - 1) * pop + i) * 3 + d] = x[i][d];
+        hist->pop_f[(size_t)(gen - 1) * pop + i] = fit[i];
+      }
+    for (int i = 0; i < pop; i++) {
+      int ex[3];
+      ex[0] = i;
+      const int r1 = pick(orc_de_hash(seed, cdp, t0, gen, i, 0), pop, ex, 1);
+      ex[1] = r1;
+      const int r2 = pick(orc_de_hash(seed, cdp, t0, gen, i, 1), pop, ex, 2);
+      ex[2] = r2;
+      const int r3 = pick(orc_de_hash(seed, cdp, t0, gen, i, 2), pop, ex, 3);
+      const int jr = (int)(orc_de_hash(seed, cdp, t0, gen, i, 3) % 3u);
+      for (int d = 0; d < 3; d++) {
+        const uint32_t u = orc_de_hash(seed, cdp, t0, gen, i, 4 + d) >> 8;
+        float v;
+        if (u < S->cr_thr || d == jr) {
+          float diff = x[r2][d] - x[r3][d];
+          float fd = S->F * diff;
+          v = x[r1][d] + fd;
+          v = fminf(fmaxf(v, lo[d]), hi[d]);
+        } else {
+          v = x[i][d];
+        }
+        tr[i][d] = v;
+      }
+      DE_EVAL(tr[i][0], tr[i][1], tr[i][2], &tf[i], &ts[i]);
+      if (hist) {
+        int32_t* pk = hist->picks + ((size_t)(gen - 1) * pop + i) * 4;
+        pk[0] = r1;
+        pk[1] = r2;
+        pk[2] = r3;
+        pk[3] = jr;
+        for (int d = 0; d < 3; d++) hist->trial_x[((size_t)(gen - 1) * pop + i) * 3 + d] = tr[i][d];
+        hist->trial_f[(size_t)(gen - 1) * pop + i] = tf[i];
+      }
+    }
+    for (int i = 0; i < pop; i++)
+      if (tf[i] >= fit[i]) {
+        x[i][0] = tr[i][0];
+        x[i][1] = tr[i][1];
+        x[i][2] = tr[i][2];
+        fit[i] = tf[i];
+        stk[i] = ts[i];
+      }
+  }
+#undef DE_EVAL
+  if (hist)
+    for (int i = 0; i < pop; i++) {
+      for (int d = 0; d < 3; d++) hist->pop_x[((size_t)S->ngen * pop + i) * 3 + d] = x[i][d];
+      hist->pop_f[(size_t)S->ngen * pop + i] = fit[i];
+    }
+  int best = 0;
+  for (int i = 1; i < pop; i++)
+    if (fit[i] > fit[best]) best = i;
+  best_x[0] = x[best][0];
+  best_x[1] = x[best][1];
+  best_x[2] = x[best][2];
+  *best_f = fit[best];
+  *best_s = stk[best];
+}
+
+/* Shared set-up of orc_crs_de and orc_de_history: box, constants, geometry. */
+typedef struct {
+  float lo[3], hi[3];
+  double *m0x, *m0y, *h0, *m2d, *md, *hd;
+  int* rows;
+  float *m2, *m, *h;
+  de_setup S;
+} de_work;
+
+static int de_open(de_work* W, int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g,
+                   const float* samples, const int32_t* nbr_off, const int32_t* nbr, double a0d, double a1d,
+                   double b0d, double b1d, double c0d, double c1d, double taud, int pop, int ngen, double Fd,
+                   double CRd, uint64_t seed) {
+  if (pop < 4 || pop > 64 || ngen < 0) return 1;
+  W->lo[0] = (float)a0d;
+  W->lo[1] = (float)b0d;
+  W->lo[2] = (float)(c0d * FACTOR);
+  W->hi[0] = (float)a1d;
+  W->hi[1] = (float)b1d;
+  W->hi[2] = (float)(c1d * FACTOR);
+  de_setup* S = &W->S;
+  S->lo = W->lo;
+  S->hi = W->hi;
+  S->F = (float)Fd;
+  S->cr_thr = (uint32_t)(CRd * 16777216.0);
+  S->seed = seed;
+  S->pop = pop;
+  S->ngen = ngen;
+  float dt = (float)dt_us;
+  dt = dt / 1000000.0f;
+  S->dt = dt;
+  S->idt = 1.0f / dt;
+  /* Window half-width as the CRS reference computes it (double). */
+  const double dtd = (double)dt_us / 1000000.0;
+  const double idtd = 1.0 / dtd;
+  S->tau = (int)(taud * idtd) > 0 ? (int)(taud * idtd) : 0;
+  S->w = 2 * S->tau + 1;
+  if (S->w > MAXW) return 1;
+  S->ns = ns;
+  S->samples = samples;
+  const int ntr = cdp_off[ncdp];
+  W->m0x = (double*)malloc(sizeof(double) * ntr);
+  W->m0y = (double*)malloc(sizeof(double) * ntr);
+  W->h0 = (double*)malloc(sizeof(double) * ntr);
+  crs_geometry(ntr, g, W->m0x, W->m0y, W->h0);
+  const int maxs = max_supergather(ncdp, cdp_off, nbr_off, nbr);
+  W->rows = (int*)malloc(sizeof(int) * maxs);
+  W->m2d = (double*)malloc(sizeof(double) * maxs);
+  W->md = (double*)malloc(sizeof(double) * maxs);
+  W->hd = (double*)malloc(sizeof(double) * maxs);
+  W->m2 = (float*)malloc(sizeof(float) * maxs);
+  W->m = (float*)malloc(sizeof(float) * maxs);
+  W->h = (float*)malloc(sizeof(float) * maxs);
+  S->rows = W->rows;
+  S->m2 = W->m2;
+  S->m = W->m;
+  S->h = W->h;
+  return 0;
+}
+
+/* Supergather of one CDP into the work arrays (FP64 geometry stored as float,
+ * the GPU exact mode's unscaled slots). */
+static void de_load_cdp(de_work* W, int cdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr) {
+  const int size = crs_supergather(cdp, cdp_off, nbr_off, nbr, W->m0x, W->m0y, W->h0, W->rows, W->m2d, W->md, W->hd);
+  for (int k = 0; k < size; k++) {
+    W->m2[k] = (float)W->m2d[k];
+    W->m[k] = (float)W->md[k];
+    W->h[k] = (float)W->hd[k];
+  }
+  W->S.size = size;
+}
+
+static void de_close(de_work* W) {
+  free(W->m0x);
+  free(W->m0y);
+  free(W->h0);
+  free(W->rows);
+  free(W->m2d);
+  free(W->md);
+  free(W->hd);
+  free(W->m2);
+  free(W->m);
+  free(W->h);
+}
+
 int orc_crs_de(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
                const int32_t* nbr_off, const int32_t* nbr, double a0d, double a1d, double b0d, double b1d, double c0d,
                double c1d, double taud, int pop, int ngen, double Fd, double CRd, uint64_t seed, float* pa,
                float* pb, float* pc, float* coher, float* stack) {
-  if (pop < 4 || pop > 64) return 1;
-  const float lo[3] = {(float)a0d, (float)b0d, (float)(c0d * FACTOR)};
-  const float hi[3] = {(float)a1d, (float)b1d, (float)(c1d * FACTOR)};
-  const float F = (float)Fd;
-  const uint32_t cr_thr = (uint32_t)(CRd * 16777216.0);
-  float dt = (float)dt_us;
-  dt = dt / 1000000.0f;
-  const float idt = 1.0f / dt;
-  /* Window half-width as the CRS reference computes it (double). */
-  const double dtd = (double)dt_us / 1000000.0;
-  const double idtd = 1.0 / dtd;
-  const int tau = (int)(taud * idtd) > 0 ? (int)(taud * idtd) : 0;
-  const int w = 2 * tau + 1;
-  if (w > MAXW) return 1;
-  const int ntr = cdp_off[ncdp];
-  double* m0x = (double*)malloc(sizeof(double) * ntr);
-  double* m0y = (double*)malloc(sizeof(double) * ntr);
-  double* h0 = (double*)malloc(sizeof(double) * ntr);
-  crs_geometry(ntr, g, m0x, m0y, h0);
-  const int maxs = max_supergather(ncdp, cdp_off, nbr_off, nbr);
-  int* rows = (int*)malloc(sizeof(int) * maxs);
-  double* m2d = (double*)malloc(sizeof(double) * maxs);
-  double* md = (double*)malloc(sizeof(double) * maxs);
-  double* hd = (double*)malloc(sizeof(double) * maxs);
-  float* m2 = (float*)malloc(sizeof(float) * maxs);
-  float* m = (float*)malloc(sizeof(float) * maxs);
-  float* h = (float*)malloc(sizeof(float) * maxs);
+  de_work W;
+  if (de_open(&W, ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0d, a1d, b0d, b1d, c0d, c1d, taud, pop, ngen,
+              Fd, CRd, seed))
+    return 1;
   for (int cdp = 0; cdp < ncdp; cdp++) {
-    const int size = crs_supergather(cdp, cdp_off, nbr_off, nbr, m0x, m0y, h0, rows, m2d, md, hd);
-    for (int k = 0; k < size; k++) {
-      m2[k] = (float)m2d[k];
-      m[k] = (float)md[k];
-      h[k] = (float)hd[k];
-    }
+    de_load_cdp(&W, cdp, cdp_off, nbr_off, nbr);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int t0 = 0; t0 < ns; t0++) {
-      float x[64][3], fit[64], stk[64], tr[64][3], tf[64], ts[64];
-      for (int i = 0; i < pop; i++) {
-        for (int d = 0; d < 3; d++) x[i][d] = lo[d] + unit(orc_de_hash(seed, cdp, t0, 0, i, d)) * (hi[d] - lo[d]);
-        de_eval(x[i][0], x[i][1], x[i][2], dt, idt, tau, w, ns, size, rows, m2, m, h, samples, t0, &fit[i], &stk[i]);
-      }
-      for (int gen = 1; gen <= ngen; gen++) {
-        for (int i = 0; i < pop; i++) {
-          int ex[3];
-          ex[0] = i;
-          const int r1 = pick(orc_de_hash(seed, cdp, t0, gen, i, 0), pop, ex, 1);
-          ex[1] = r1;
-          const int r2 = pick(orc_de_hash(seed, cdp, t0, gen, i, 1), pop, ex, 2);
-          ex[2] = r2;
-          const int r3 = pick(orc_de_hash(seed, cdp, t0, gen, i, 2), pop, ex, 3);
-          const int jr = (int)(orc_de_hash(seed, cdp, t0, gen, i, 3) % 3u);
-          for (int d = 0; d < 3; d++) {
-            const uint32_t u = orc_de_hash(seed, cdp, t0, gen, i, 4 + d) >> 8;
-            float v;
-            if (u < cr_thr || d == jr) {
-              float diff = x[r2][d] - x[r3][d];
-              float fd = F * diff;
-              v = x[r1][d] + fd;
-              v = fminf(fmaxf(v, lo[d]), hi[d]);
-            } else {
-              v = x[i][d];
-            }
-            tr[i][d] = v;
-          }
-          de_eval(tr[i][0], tr[i][1], tr[i][2], dt, idt, tau, w, ns, size, rows, m2, m, h, samples, t0, &tf[i],
-                  &ts[i]);
-        }
-        for (int i = 0; i < pop; i++)
-          if (tf[i] >= fit[i]) {
-            x[i][0] = tr[i][0];
-            x[i][1] = tr[i][1];
-            x[i][2] = tr[i][2];
-            fit[i] = tf[i];
-            stk[i] = ts[i];
-          }
-      }
-      int best = 0;
-      for (int i = 1; i < pop; i++)
-        if (fit[i] > fit[best]) best = i;
+      float bx[3], bf, bs;
+      de_point(&W.S, cdp, t0, bx, &bf, &bs, NULL);
       const size_t o = (size_t)cdp * ns + t0;
-      pa[o] = x[best][0];
-      pb[o] = x[best][1];
-      pc[o] = (float)((double)x[best][2] / FACTOR);
-      coher[o] = fit[best];
-      stack[o] = stk[best];
+      pa[o] = bx[0];
+      pb[o] = bx[1];
+      pc[o] = (float)((double)bx[2] / FACTOR);
+      coher[o] = bf;
+      stack[o] = bs;
     }
   }
-  free(m0x);
-  free(m0y);
-  free(h0);
-  free(rows);
-  free(m2d);
-  free(md);
-  free(hd);
-  free(m2);
-  free(m);
-  free(h);
+  de_close(&W);
+  return 0;
+}
+
+int orc_de_history(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                   const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, double b0, double b1, double c0,
+                   double c1, double tau, int pop, int ngen, double F, double CR, uint64_t seed, int cdp, int t0,
+                   float* pop_x, float* pop_f, float* trial_x, float* trial_f, int32_t* picks, float* best) {
+  if (cdp < 0 || cdp >= ncdp || t0 < 0 || t0 >= ns) return 1;
+  de_work W;
+  if (de_open(&W, ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0, a1, b0, b1, c0, c1, tau, pop, ngen, F, CR,
+              seed))
+    return 1;
+  de_load_cdp(&W, cdp, cdp_off, nbr_off, nbr);
+  de_hist H = {pop_x, pop_f, trial_x, trial_f, picks};
+  de_point(&W.S, cdp, t0, best, best + 3, best + 4, &H);
+  de_close(&W);
   return 0;
 }

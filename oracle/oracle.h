@@ -53,6 +53,18 @@ int orc_crs_de(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_ge
                double c1, double tau, int pop, int ngen, double F, double CR, uint64_t seed, float* pa, float* pb,
                float* pc, float* coher, float* stack);
 
+/* The same search at one (cdp, t0), with its whole trajectory: pop_x[g]
+ * [pop][3] (c scaled by 1e6) and pop_f[g][pop] = the population and its
+ * fitness after g generations (g in [0, ngen]; 0 = the initial one); the
+ * trials of generation g + 1, made from pop_x[g], in trial_x[g][pop][3] with
+ * fitness trial_f[g][pop] and draws picks[g][pop][4] = {r1, r2, r3, jr}
+ * (g in [0, ngen)).  best[5] = the reported a, b, c (scaled), fitness and
+ * stack. */
+int orc_de_history(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                   const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, double b0, double b1, double c0,
+                   double c1, double tau, int pop, int ngen, double F, double CR, uint64_t seed, int cdp, int t0,
+                   float* pop_x, float* pop_f, float* trial_x, float* trial_f, int32_t* picks, float* best);
+
 /* Counter-based uniform draw shared with the GPU DE kernel (for tests). */
 uint32_t orc_de_hash(uint64_t seed, uint32_t cdp, uint32_t t0, uint32_t gen, uint32_t member, uint32_t draw);
 

@@ -33,6 +33,8 @@ def lib():
         L.orc_cmp_f64.argtypes = [i, P, i, i, P, P, d, d, i, d, P, P, P, P]
         L.orc_crs_eval_f64.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, i, P, P]
         L.orc_crs_de.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, d, d, d, i, i, d, d, C.c_uint64, P, P, P, P, P]
+        L.orc_de_history.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, d, d, d, i, i, d, d, C.c_uint64, i, i,
+                                     P, P, P, P, P, P]
         L.orc_de_hash.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_de_hash.restype = C.c_uint32
         _lib = L
@@ -120,6 +122,28 @@ def crs_de(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, b0, b1, c0, c1, 
                           _p(out["coher"]), _p(out["stack"]))
     assert rc == 0
     return out
+
+
+def de_history(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, b0, b1, c0, c1, tau, pop, ngen, F, CR, seed,
+               cdp, t0):
+    """Trajectory of the CRS-DE twin at one (cdp, t0): pop_x [ngen+1][pop][3]
+    and pop_f [ngen+1][pop] (population after g generations, c scaled by 1e6),
+    trial_x [ngen][pop][3], trial_f [ngen][pop], picks [ngen][pop][4] = (r1, r2,
+    r3, jr) of the trials made from pop_x[g], and best = (a, b, c scaled,
+    fitness, stack) as reported."""
+    cdp_off, samples, geom = _c(cdp_off, np.int32), _c(samples, np.float32), _geom(geom)
+    nbr_off, nbr = _c(nbr_off, np.int32), _c(nbr if len(nbr) else [0], np.int32)
+    ncdp, ns = len(cdp_off) - 1, samples.shape[1]
+    h = {"pop_x": np.zeros((ngen + 1, pop, 3), np.float32), "pop_f": np.zeros((ngen + 1, pop), np.float32),
+         "trial_x": np.zeros((max(ngen, 1), pop, 3), np.float32), "trial_f": np.zeros((max(ngen, 1), pop), np.float32),
+         "picks": np.zeros((max(ngen, 1), pop, 4), np.int32), "best": np.zeros(5, np.float32)}
+    rc = lib().orc_de_history(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, b0,
+                              b1, c0, c1, tau, pop, ngen, F, CR, seed, cdp, t0, _p(h["pop_x"]), _p(h["pop_f"]),
+                              _p(h["trial_x"]), _p(h["trial_f"]), _p(h["picks"]), _p(h["best"]))
+    assert rc == 0
+    for k in ("trial_x", "trial_f", "picks"):
+        h[k] = h[k][:ngen]
+    return h
 
 
 def de_hash(seed, cdp, t0, gen, member, draw):

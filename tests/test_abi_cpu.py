@@ -85,21 +85,27 @@ def test_synth_is_deterministic_and_binned(sb):
 
 
 def test_keep_trace_matches_float_and_double_rules(sb):
-    """CMP compares in float, CRS in double: a half-offset just above aph in
-    double but equal in float is kept by CMP and dropped by CRS."""
+    """CMP squares and sums the half-offsets in float, CRS in double
+    (CMP/OpenMP/src/su_gather.cpp:44-47 with real = float; the same lines of
+    CRS/OpenMP/src/su_gather.cpp with real = double).  Constructed case, scalco
+    0 (factor 1), aph 5000: hx = 5000, hy = 1 gives hx^2 + hy^2 = 25 000 001,
+    above aph^2 = 25 000 000 exactly, but in float the sum lies halfway between
+    the neighbours 25 000 000 and 25 000 002 (ulp 2 above 2^24) and rounds to
+    the even one, 25 000 000, which is not above aph^2: CMP keeps the trace,
+    CRS drops it."""
     L = sb.lib()
-    # scalco -100: coordinates in cm; half-offset = (gx - sx) / 200 m.
-    found = 0
-    for d in range(40000, 400000, 7):
-        g = np.array([(1, -100, 0, 0, d, 1)], sb.GEOM_DTYPE)
-        hx = np.float32(np.float32(np.float32(-0.01) * np.float32(-1) * -1) * 0) if False else None
-        k_cmp = L.sb_keep_trace(g.ctypes.data, sb.METHOD_CMP, 1000)
-        k_crs = L.sb_keep_trace(g.ctypes.data, sb.METHOD_CRS, 1000)
-        if k_cmp != k_crs:
-            found += 1
-    # The float and double paths agree except near the boundary; both are
-    # exercised end to end by the golden aph fixture.
-    assert found < 50
+
+    def keep(gx, gy, method):
+        g = np.array([(1, 0, 0, 0, gx, gy)], sb.GEOM_DTYPE)  # cdp, scalco, sx, sy, gx, gy
+        return L.sb_keep_trace(g.ctypes.data, method, 5000)
+
+    # half-offsets (gx - sx) / 2 = 5000 and (gy - sy) / 2 = 1
+    assert keep(10000, 2, sb.METHOD_CMP) == 1
+    assert keep(10000, 2, sb.METHOD_CRS) == 0
+    # on the aperture exactly (not above): both keep; hy = 2 (sum 25 000 004,
+    # exact in float): both drop
+    assert keep(10000, 0, sb.METHOD_CMP) == 1 and keep(10000, 0, sb.METHOD_CRS) == 1
+    assert keep(10000, 4, sb.METHOD_CMP) == 0 and keep(10000, 4, sb.METHOD_CRS) == 0
 
 
 def test_neighbors_match_bruteforce(sb):
