@@ -1,9 +1,11 @@
 """HIP path at production sizes and the reduced-precision stores.
 
 Tolerances (written here, checked below):
-  FP32 store, SB_MODE_FAST: winning index = oracle's except where the
-      oracle's top two semblances are within 1e-5 (relative); coherence and
-      stack then bit-identical to the reference-precision oracle.
+  FP32 store, SB_MODE_FAST: winning index = the reference's except where the
+      reference's top two semblances are within 1e-5 (tests/parity.py; a
+      capped, counted number of picks the FP64 oracle ranks at least as
+      high); coherence and stack at every pick bit-identical to the
+      reference-precision oracle's values for that pick.
   FP16 store: semblance at the reported (t0, c) within 2e-3 relative of the
       FP64 oracle (semblances >= 0.05), stack within 2e-3 of the CDP's
       amplitude scale (max |stack| of the FP64 oracle).
@@ -11,6 +13,8 @@ Tolerances (written here, checked below):
 """
 import numpy as np
 import pytest
+
+from parity import check_fast_picks
 
 pytestmark = pytest.mark.gpu
 
@@ -22,20 +26,6 @@ def subset(cdp_off, geom, samples, cdps):
     rows = np.concatenate([np.arange(cdp_off[i], cdp_off[i + 1]) for i in cdps])
     off = np.concatenate([[0], np.cumsum(np.diff(cdp_off)[cdps])]).astype(np.int32)
     return off, geom[rows], samples[rows]
-
-
-def near_tie(row, pick, tol=1e-5):
-    """Top-two semblances (values in [0, 1]) within tol of each other."""
-    return pick >= 0 and float(np.max(row)) - float(row[pick]) <= tol
-
-
-def explained(row32, row64, ref_pick, our_pick, tol=1e-5):
-    """A pick that differs from the reference's is explained by a near-tie in
-    the reference's own values, or by the reference's FP32 rounding: the FP64
-    oracle ranks our pick at least as high as the reference's."""
-    if our_pick < 0 or ref_pick < 0:
-        return False
-    return near_tie(row32, our_pick, tol) or row64[our_pick] >= row64[ref_pick] - 1e-9
 
 
 def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx):
@@ -63,18 +53,8 @@ def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx):
     ref = orc.cmp(off, gs, ss, 2000, C0, C1, 256, 0.002, panel=True)
     ref64 = orc.cmp_f64(off, gs, ss, 2000, C0, C1, 256, 0.002, panel=True)
     mine = {k: v[pick] for k, v in out.items()}
-    diff = np.argwhere(mine["index"] != ref["index"])
-    gaps = [float(np.max(ref["panel_sem"][c, t]) - ref["panel_sem"][c, t, max(mine["index"][c, t], 0)])
-            for c, t in diff]
-    print("fast vs reference: %d/%d picks differ, top-two gaps (abs) %s" %
-          (len(diff), mine["index"].size, np.round(gaps, 8).tolist()[:20]))
-    for cdp, t0 in diff:
-        assert explained(ref["panel_sem"][cdp, t0], ref64["panel_sem"][cdp, t0], ref["index"][cdp, t0],
-                         mine["index"][cdp, t0]), (pick[cdp], t0, gaps)
-    same = mine["index"] == ref["index"]
-    assert same.mean() > 0.999
-    for k in ("coher", "stack", "c"):
-        assert np.array_equal(mine[k][same], ref[k][same]), k
+    ndiff, _ = check_fast_picks("cmp 2000 CDPs fold 30-120 ns 1501 nc 256", ref, mine, ref64, params=("c",))
+    assert ndiff <= 1e-3 * mine["index"].size
 
 
 @pytest.mark.parametrize("store,tol_sem,tol_stk", [("f16", 2e-3, 2e-3), ("bf16", 2e-2, 2e-2)])
@@ -181,17 +161,10 @@ def test_crs_fast_larger_gather(sb, orc, gpu_ctx):
     g.close()
     pick = [0, 5, 11]
     ref = orc.crs(cdp_off, geom, samples, 4000, nbr_off, nbr, -0.7e-3, 0.7e-3, 11, -1e-7, 1e-7, 11, 1.975e-7,
-                  1.384e-6, 11, 0.004, panel=False)
-    for cdp in pick:
-        diff = np.nonzero(out["index"][cdp] != ref["index"][cdp])[0]
-        same = out["index"][cdp] == ref["index"][cdp]
-        assert same.mean() > 0.98, cdp
-        for k in ("a", "b", "c", "coher", "stack"):
-            assert np.array_equal(out[k][cdp][same], ref[k][cdp][same]), (cdp, k)
-        for t0 in diff:  # near-ties only: the two picks' double semblances within 1e-5
-            sa = ref["coher"][cdp, t0]
-            sb_ = out["coher"][cdp, t0]
-            assert abs(sa - sb_) <= 1e-5, (cdp, t0, sa, sb_)
+                  1.384e-6, 11, 0.004, panel=True)
+    ref = {k: v[pick] for k, v in ref.items()}
+    mine = {k: v[pick] for k, v in out.items()}
+    check_fast_picks("crs 12 CDPs 11^3", ref, mine, None, params=("a", "b", "c"))
 
 
 def _crs_line(sb, ncdp=6):
