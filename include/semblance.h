@@ -224,6 +224,38 @@ int sb_crs_run_range(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, in
 int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, int cdp_begin, int cdp_end,
                         sb_result* r);
 
+/* Caller-owned device arrays of (cdp_end - cdp_begin) x ns results, laid out
+ * as sb_host_result's; all six are written. */
+typedef struct {
+  int32_t* index;
+  float* a;
+  float* b;
+  float* c;
+  float* coher;
+  float* stack;
+} sb_device_result;
+
+/* Bytes of the static device scratch behind sb_cmp_run_noalloc (per device,
+ * held from the first sb_ctx_create on that device until the process ends):
+ * 1 MiB of work table (65 536 work items) and 3 MiB of arg-max keys (393 216
+ * (CDP, t0) results). */
+#define SB_NOALLOC_SCRATCH_BYTES (4u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sb_cmp_run_range with the results written straight to
+ * caller-owned device arrays and every scratch buffer (work table, arg-max
+ * keys) in a static device array of SB_NOALLOC_SCRATCH_BYTES, so the call
+ * allocates nothing -- no hipMalloc, hipFree or host staging; the work table
+ * is built on the device.  Same kernels and results as sb_cmp_run_range.
+ * Searches whose work table or keys exceed the scratch fail with
+ * SB_ERR_UNSUPPORTED (e.g. more than 393 216 CDP x ns results).  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one device
+ * (two ctxs, two host threads) race, so serialise them.  Runs on the ctx
+ * stream; the SB_OPT_PERSISTENT option does not apply. */
+int sb_cmp_run_noalloc(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cdp_begin, int cdp_end,
+                       const sb_device_result* out);
+
 /* Convenience: search and download in one call. */
 int sb_cmp(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_host_result* out);
 int sb_crs(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_host_result* out);

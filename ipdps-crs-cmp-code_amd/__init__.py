@@ -55,6 +55,11 @@ class HostResult(C.Structure):
                 ("c", C.POINTER(C.c_float)), ("coher", C.POINTER(C.c_float)), ("stack", C.POINTER(C.c_float))]
 
 
+class DeviceArrays(C.Structure):
+    _fields_ = [("index", C.c_void_p), ("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p),
+                ("coher", C.c_void_p), ("stack", C.c_void_p)]
+
+
 class Timing(C.Structure):
     _fields_ = [("search_ms", C.c_double), ("finalize_ms", C.c_double), ("total_ms", C.c_double),
                 ("tuples", C.c_double), ("grid_blocks", C.c_int), ("block_threads", C.c_int),
@@ -95,7 +100,7 @@ EXPORTS = [
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
     "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
-    "sb_search_streamed",
+    "sb_search_streamed", "sb_cmp_run_noalloc",
 ]
 
 
@@ -174,6 +179,7 @@ def lib():
                                          C.c_int, i32p, i32p, C.POINTER(HostResult), C.POINTER(P)]
         L.sb_graph_destroy.argtypes = [P]
         L.sb_graph_destroy.restype = None
+        L.sb_cmp_run_noalloc.argtypes = [P, P, C.POINTER(CmpParams), C.c_int, C.c_int, C.POINTER(DeviceArrays)]
         _lib = L
     return _lib
 

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
@@ -211,6 +212,56 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Work table built on the device (the allocation-free entry point has no host
+// staging): item slot b serves XCD b % 8 and takes item (b % 8) * per + b / 8
+// of the (CDP, tile, chunk)-ordered list, so each XCD runs a contiguous run of
+// CDPs (their blocks share that XCD's L2); slots past the list are no-ops.
+__global__ void k_make_work(WorkItem* __restrict__ w, int nblocks, int cdp0, int ncdp, int ntiles, int nchunks,
+                            int pchunk, int npar) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const int64_t n = (int64_t)ncdp * ntiles * nchunks;
+  const int64_t per = (n + 7) / 8;
+  const int64_t i = (int64_t)(b % 8) * per + b / 8;
+  WorkItem it{-1, 0, 0, 0};
+  if (i < n) {
+    const int64_t ct = i / nchunks;
+    const int k = (int)(i - ct * nchunks);
+    it.cdp = cdp0 + (int)(ct / ntiles);
+    it.tile = (int)(ct - (ct / ntiles) * ntiles);
+    it.p0 = k * pchunk;
+    it.p1 = min(npar, (k + 1) * pchunk);
+  }
+  w[b] = it;
+}
+
+// The allocation-free entry point's scratch: one copy per device, alive until
+// the process ends.  Host code never uses the name itself (on the host it is
+// a stand-in, not a device address): scratch_address looks up the device
+// address once per device and every launch takes that pointer.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
 }
 
 // ---------------------------------------------- reference-order evaluation
@@ -1082,6 +1133,14 @@ hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int s
     hipLaunchKernelGGL(k_quantize<SB_STORE_F16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
   else
     hipLaunchKernelGGL(k_quantize<SB_STORE_BF16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_make_work(WorkItem* w, int nblocks, int cdp0, int ncdp, int ntiles, int nchunks, int pchunk,
+                            int npar, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_make_work, dim3((nblocks + 255) / 256), dim3(256), 0, st, w, nblocks, cdp0, ncdp, ntiles,
+                     nchunks, pchunk, npar);
   return SB_CHECK_LAUNCH();
 }
 

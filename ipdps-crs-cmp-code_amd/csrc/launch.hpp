@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include "device_types.hpp"
+#include "semblance.h"
 
 namespace sb {
 
@@ -29,6 +30,24 @@ extern const int kNumFastVariants;
 constexpr int kDefaultFastVariant = 5;
 
 bool fast_supported(int w);
+
+// Static device scratch of the allocation-free entry point
+// (sb_cmp_run_noalloc), one copy per device: the work table (16-byte items)
+// in its first kScratchKeysOffset bytes, the arg-max keys (8 bytes per
+// (CDP, t0)) after them.
+constexpr size_t kScratchBytes = SB_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchKeysOffset = 1u << 20;
+constexpr int64_t kScratchWorkItems = (int64_t)(kScratchKeysOffset / sizeof(WorkItem));
+constexpr int64_t kScratchKeys = (int64_t)((kScratchBytes - kScratchKeysOffset) / 8);
+// Device address of the scratch on `device` (which must be current), looked
+// up with hipGetSymbolAddress on the first call for that device and kept.
+hipError_t scratch_address(int device, void** out);
+// Work table of CDPs [cdp0, cdp0 + ncdp) built on the device: nblocks (a
+// multiple of 8) items, every (CDP, t0 tile, parameter chunk) once, in that
+// order, the XCD of block b (b % 8) taking its own contiguous eighth of the
+// list; the rest no-ops.
+hipError_t launch_make_work(WorkItem* w, int nblocks, int cdp0, int ncdp, int ntiles, int nchunks, int pchunk,
+                            int npar, hipStream_t st);
 
 hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
                               hipStream_t st);

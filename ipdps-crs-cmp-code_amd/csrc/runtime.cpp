@@ -193,6 +193,9 @@ extern "C" int sb_ctx_create(int device, size_t hbm_budget, sb_ctx** out) {
   hipDeviceProp_t prop;
   HIPR(hipGetDeviceProperties(&prop, device));
   c->num_cus = prop.multiProcessorCount;
+  // The allocation-free entry point's static scratch: its device address on
+  // this device (looked up once per device, with the device current).
+  HIPR(scratch_address(device, &c->scratch));
   *out = c.release();
   return SB_OK;
 }
@@ -678,8 +681,16 @@ bool lds_fits(const sb_gather* g, const SearchArgs& a, const FastVariant& v, int
   return smp >= 0 && (int64_t)smp + 4096 <= (int64_t)v.lds_kb * 1024;
 }
 
-int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int mode, double params_per_tuple,
-               const std::vector<int64_t>& S, int cb, int ce) {
+// Where a search's scratch lives: the ctx's cached work table and the
+// result's key array (allocated through the ctx), or, for the
+// allocation-free entry point, the static device array: the work table built
+// on the device in its first bytes, the keys after it.
+struct Scratch {
+  void* base = nullptr;  // null: the ctx's buffers
+};
+
+int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double params_per_tuple,
+               const std::vector<int64_t>& S, int cb, int ce, Scratch sc = Scratch{}) {
   HIPR(hipSetDevice(ctx->device));
   std::lock_guard<std::mutex> lk(ctx->run_mu);
   int rc;
@@ -697,6 +708,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   // Reference arithmetic on the tiled layout (keys + winner re-evaluation)
   // where it is compiled; otherwise one thread per (CDP, t0).
   const bool tiled_exact = mode == SB_MODE_EXACT && exact_tiled_supported(a.method, store, a.w);
+  // The work table's tiling: family (for the cache key), t0 tiles and
+  // parameters per workgroup.
+  int family, ntiles, npar_w, pchunk;
   if (mode == SB_MODE_FAST) {
     variant = ctx->fast_variant;
     v = kFastVariants[variant];
@@ -704,39 +718,57 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
       lds = true;
       a.lds_bytes = v.lds_kb * 1024;
     }
-    {
-      // The global-memory kernel runs the first global shape when the LDS
-      // slabs do not apply.  Both kernels read the same work table.
-      if (v.lds_kb > 0 && !lds) {
-        variant = 0;
-        v = kFastVariants[0];
-      }
-      const int tile = 64 * v.R;
-      const int ntiles = (g->ns + tile - 1) / tile;
-      const int pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override
-                         : lds ? lds_default_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC)
-                               : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
-      const std::string key = work_key(g, a.method, 0, ntiles, a.npar, pchunk, cb, ce);
-      if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+    // The global-memory kernel runs the first global shape when the LDS
+    // slabs do not apply.  Both kernels read the same work table.
+    if (v.lds_kb > 0 && !lds) {
+      variant = 0;
+      v = kFastVariants[0];
     }
+    family = 0;
+    ntiles = (g->ns + 64 * v.R - 1) / (64 * v.R);
+    npar_w = a.npar;
+    pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override
+             : lds ? lds_default_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC)
+                   : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
   } else if (tiled_exact) {
-    const int ntiles = (g->ns + 63) / 64;
-    const int pchunk = a.method == SB_METHOD_CRS ? a.npar : choose_pchunk(ctx, ce - cb, ntiles, a.npar, 16);
-    const std::string key = work_key(g, a.method, 2, ntiles, a.npar, pchunk, cb, ce);
-    if ((rc = build_work(ctx, cost, cb, ce, ntiles, a.npar, pchunk, &nblocks, key))) return rc;
+    family = 2;
+    ntiles = (g->ns + 63) / 64;
+    npar_w = a.npar;
+    pchunk = a.method == SB_METHOD_CRS ? a.npar : choose_pchunk(ctx, ce - cb, ntiles, a.npar, 16);
   } else {
-    const int ntiles = (g->ns + 63) / 64;
-    const std::string key = work_key(g, a.method, 1, ntiles, 1, 1, cb, ce);
-    if ((rc = build_work(ctx, cost, cb, ce, ntiles, 1, 1, &nblocks, key))) return rc;
+    family = 1;
+    ntiles = (g->ns + 63) / 64;
+    npar_w = 1;
+    pchunk = 1;
   }
-  a.work = work_table(ctx);
-  a.nwork = nblocks;
   const int64_t nout = (int64_t)(ce - cb) * g->ns;
+  if (!sc.base) {
+    const std::string key = work_key(g, a.method, family, ntiles, npar_w, pchunk, cb, ce);
+    if ((rc = build_work(ctx, cost, cb, ce, ntiles, npar_w, pchunk, &nblocks, key))) return rc;
+    a.work = work_table(ctx);
+  } else {
+    const int nchunks = std::max(1, (npar_w + pchunk - 1) / pchunk);
+    const int64_t items = (int64_t)(ce - cb) * ntiles * nchunks;
+    const int64_t nb = (items + 7) / 8 * 8;
+    if (nb > kScratchWorkItems || nout > kScratchKeys)
+      return fail(SB_ERR_UNSUPPORTED, "search too large for the static scratch: " + std::to_string(nb) +
+                                          " work items (max " + std::to_string(kScratchWorkItems) + "), " +
+                                          std::to_string(nout) + " (CDP, t0) keys (max " +
+                                          std::to_string(kScratchKeys) + ")");
+    nblocks = (int)nb;
+    WorkItem* w = (WorkItem*)sc.base;
+    a.keys = (unsigned long long*)((char*)sc.base + kScratchKeysOffset);
+    const hipError_t e = launch_make_work(w, nblocks, cb, ce - cb, ntiles, nchunks, pchunk, npar_w, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "work table");
+    a.work = w;
+  }
+  a.nwork = nblocks;
   HIPR(record(ctx, 0));
   hipError_t e = hipSuccess;
-  if (mode == SB_MODE_FAST || tiled_exact) e = launch_fill_u64(r->keys, nout, 0ull, ctx->stream);
+  if (mode == SB_MODE_FAST || tiled_exact) e = launch_fill_u64(a.keys, nout, 0ull, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fill");
-  if (mode == SB_MODE_FAST && !lds && ctx->persistent) {
+  const bool persistent = mode == SB_MODE_FAST && !lds && ctx->persistent && !sc.base;
+  if (persistent) {
     // Persistent grid: 8 per-XCD item counters, cleared by a kernel (not a
     // memset, so that the sequence can be captured into a graph).
     if (!ctx->queue_buf && (rc = ctx_alloc(ctx, 64, &ctx->queue_buf))) return rc;
@@ -763,7 +795,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
-  ctx->last.block_threads = mode == SB_MODE_FAST || tiled_exact ? 256 : 64;
+  ctx->last.block_threads = mode == SB_MODE_FAST || tiled_exact ? (lds ? 64 * v.waves : 256) : 64;
   ctx->last.variant = variant;
   ctx->last.lds_staged = lds ? 1 : 0;
   ctx->timing_pending = true;
@@ -771,26 +803,16 @@ int run_search(sb_ctx* ctx, const sb_gather* g, sb_result* r, SearchArgs& a, int
   return SB_OK;
 }
 
-}  // namespace
-
-extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_result* r) {
-  if (!g) return fail(SB_ERR_INVALID, "sb_cmp_run: null gather");
-  return sb_cmp_run_range(ctx, g, p, 0, g->ncdp, r);
-}
-
-extern "C" int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cb, int ce,
-                                sb_result* r) {
-  if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_cmp_run: null argument");
-  int rc;
-  if ((rc = check_range(ctx, g, r, cb, ce))) return rc;
+// CMP parameter set-up exactly as CMP/OpenMP/src/main.cpp:68-110 (float):
+// grid, window and the fast search's t^2 bound; *mode = the arithmetic that
+// runs (a negative C range runs in reference arithmetic).
+int cmp_setup(const sb_gather* g, const sb_cmp_params* p, SearchArgs& a, int* mode) {
   if (p->nc <= 0) return fail(SB_ERR_INVALID, "nc must be positive");
   if (p->mode != SB_MODE_FAST && p->mode != SB_MODE_EXACT) return fail(SB_ERR_INVALID, "bad mode");
-  SearchArgs a;
-  base_args(g, r, cb, a);
   a.method = SB_METHOD_CMP;
   a.slot_off = g->d_cdp_off;
   a.slots = g->d_cmp_slots;
-  // CMP/OpenMP/src/main.cpp:68-110, all in float: c0 = stof(c0) * 1e6, ...
+  // c0 = stof(c0) * 1e6, ...
   const float c0 = (float)((double)(float)p->c0 * 1e6);
   const float c1 = (float)((double)(float)p->c1 * 1e6);
   const float itau = (float)p->tau;
@@ -816,12 +838,68 @@ extern "C" int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_pa
     const int hi = g->ns - tau - 1;
     a.hi2 = hi > 0 ? (float)((double)hi * (double)hi * (1.0 - 0x1p-20)) : 0.f;
   }
-  std::vector<int64_t> S(g->ncdp);
-  for (int i = 0; i < g->ncdp; i++) S[i] = g->cdp_off[i + 1] - g->cdp_off[i];
   // The fast kernel assumes t0^2 + C h^2 >= 0; a negative C range (NaN
   // traveltimes, all invalid in the reference) runs in reference arithmetic.
-  const int mode = (c0 < 0.f || c1 < 0.f) ? SB_MODE_EXACT : p->mode;
-  return run_search(ctx, g, r, a, mode, (double)p->nc, S, cb, ce);
+  *mode = (c0 < 0.f || c1 < 0.f) ? SB_MODE_EXACT : p->mode;
+  return SB_OK;
+}
+
+std::vector<int64_t> cmp_cost(const sb_gather* g) {
+  std::vector<int64_t> S(g->ncdp);
+  for (int i = 0; i < g->ncdp; i++) S[i] = g->cdp_off[i + 1] - g->cdp_off[i];
+  return S;
+}
+
+}  // namespace
+
+extern "C" int sb_cmp_run(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, sb_result* r) {
+  if (!g) return fail(SB_ERR_INVALID, "sb_cmp_run: null gather");
+  return sb_cmp_run_range(ctx, g, p, 0, g->ncdp, r);
+}
+
+extern "C" int sb_cmp_run_range(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cb, int ce,
+                                sb_result* r) {
+  if (!ctx || !g || !p || !r) return fail(SB_ERR_INVALID, "sb_cmp_run: null argument");
+  int rc;
+  if ((rc = check_range(ctx, g, r, cb, ce))) return rc;
+  SearchArgs a;
+  base_args(g, r, cb, a);
+  int mode;
+  if ((rc = cmp_setup(g, p, a, &mode))) return rc;
+  return run_search(ctx, g, a, mode, (double)p->nc, cmp_cost(g), cb, ce);
+}
+
+extern "C" int sb_cmp_run_noalloc(sb_ctx* ctx, const sb_gather* g, const sb_cmp_params* p, int cb, int ce,
+                                  const sb_device_result* out) {
+  if (!ctx || !g || !p || !out) return fail(SB_ERR_INVALID, "sb_cmp_run_noalloc: null argument");
+  if (!out->index || !out->a || !out->b || !out->c || !out->coher || !out->stack)
+    return fail(SB_ERR_INVALID, "sb_cmp_run_noalloc: every output array is required");
+  if (cb < 0 || ce > g->ncdp || cb >= ce) return fail(SB_ERR_INVALID, "bad CDP range");
+  if (g->ctx->device != ctx->device) return fail(SB_ERR_INVALID, "gather on another device");
+  if (!ctx->scratch) return fail(SB_ERR_STATE, "static scratch not initialised on this device");
+  int rc;
+  SearchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.cdp_base = cb;
+  a.nout = ce - cb;
+  a.samples = g->d_samples;
+  a.scale = g->d_scale;
+  a.stride = g->stride;
+  a.cdp_off = g->d_cdp_off;
+  a.h_ref32 = g->d_h_ref32;
+  a.ncdp = g->ncdp;
+  a.ns = g->ns;
+  a.out_index = out->index;
+  a.out_a = out->a;
+  a.out_b = out->b;
+  a.out_c = out->c;
+  a.out_coher = out->coher;
+  a.out_stack = out->stack;
+  int mode;
+  if ((rc = cmp_setup(g, p, a, &mode))) return rc;
+  Scratch sc;
+  sc.base = ctx->scratch;
+  return run_search(ctx, g, a, mode, (double)p->nc, cmp_cost(g), cb, ce, sc);
 }
 
 extern "C" int sb_crs_run(sb_ctx* ctx, const sb_gather* g, const sb_crs_params* p, sb_result* r) {
@@ -877,7 +955,7 @@ extern "C" int sb_crs_run_range(sb_ctx* ctx, const sb_gather* g, const sb_crs_pa
   a.idt_d = idt;
   std::vector<int64_t> S(g->ncdp);
   for (int i = 0; i < g->ncdp; i++) S[i] = g->crs_slot_off[i + 1] - g->crs_slot_off[i];
-  return run_search(ctx, g, r, a, p->mode, (double)a.npar, S, cb, ce);
+  return run_search(ctx, g, a, p->mode, (double)a.npar, S, cb, ce);
 }
 
 extern "C" int sb_crs_de_run(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_result* r) {

@@ -48,6 +48,7 @@ struct sb_ctx {
   void* queue_buf = nullptr;   // persistent-grid item counters
   int persistent = 0;          // SB_OPT_PERSISTENT
   int stream_rows = 0;         // SB_OPT_STREAM_ROWS (0: automatic)
+  void* scratch = nullptr;     // device address of the static scratch on this device
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured
 };
