@@ -210,6 +210,15 @@ int orc_crs(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom*
             const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1, int nb,
             double c0, double c1, int nc, double taud, int32_t* index, float* pa, float* pb, float* pc,
             float* coher, float* stack, float* panel_sem, float* panel_stk) {
+  return orc_crs_range(ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0, c1, nc, taud,
+                       0, ncdp, index, pa, pb, pc, coher, stack, panel_sem, panel_stk);
+}
+
+int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                  const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1,
+                  int nb, double c0, double c1, int nc, double taud, int cdp_begin, int cdp_end, int32_t* index,
+                  float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem, float* panel_stk) {
+  if (cdp_begin < 0 || cdp_end > ncdp || cdp_begin >= cdp_end) return 1;
   c0 = c0 * FACTOR;
   c1 = c1 * FACTOR;
   const double inc_a = (a1 - a0) * (1.0 / (double)na);
@@ -238,8 +247,9 @@ int orc_crs(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom*
     pars[3 * i + 1] = b0 + inc_b * ((i / nc) % nb);
     pars[3 * i + 2] = c0 + inc_c * (i % nc);
   }
-  for (int cdp = 0; cdp < ncdp; cdp++) {
+  for (int cdp = cdp_begin; cdp < cdp_end; cdp++) {
     const int size = crs_supergather(cdp, cdp_off, nbr_off, nbr, m0x, m0y, h0, rows, m2, m, h);
+    const size_t row0 = (size_t)(cdp - cdp_begin) * ns;
     double stt00 = 0, sem00 = 0;
     crs_eval(pars[0], pars[1], pars[2], dt, idt, tau, w, ns, size, rows, m2, m, h, samples, 0, &sem00, &stt00);
 #pragma omp parallel for schedule(static)
@@ -250,8 +260,8 @@ int orc_crs(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom*
         double sem, stk;
         crs_eval(pars[3 * p], pars[3 * p + 1], pars[3 * p + 2], dt, idt, tau, w, ns, size, rows, m2, m, h, samples,
                  t0, &sem, &stk);
-        if (panel_sem) panel_sem[((size_t)cdp * ns + t0) * npar + p] = (float)sem;
-        if (panel_stk) panel_stk[((size_t)cdp * ns + t0) * npar + p] = (float)stk;
+        if (panel_sem) panel_sem[(row0 + t0) * npar + p] = (float)sem;
+        if (panel_stk) panel_stk[(row0 + t0) * npar + p] = (float)stk;
         if (sem > max_sem) {
           max_sem = sem;
           max_par = p;
@@ -263,7 +273,7 @@ int orc_crs(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom*
         max_par = 0;
         best_stk = stt00;
       }
-      const size_t o = (size_t)cdp * ns + t0;
+      const size_t o = row0 + t0;
       index[o] = max_par;
       coher[o] = (float)max_sem;
       stack[o] = (float)best_stk;
@@ -665,17 +675,26 @@ int orc_crs_de(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_ge
                const int32_t* nbr_off, const int32_t* nbr, double a0d, double a1d, double b0d, double b1d, double c0d,
                double c1d, double taud, int pop, int ngen, double Fd, double CRd, uint64_t seed, float* pa,
                float* pb, float* pc, float* coher, float* stack) {
+  return orc_crs_de_range(ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0d, a1d, b0d, b1d, c0d, c1d, taud, pop,
+                          ngen, Fd, CRd, seed, 0, ncdp, pa, pb, pc, coher, stack);
+}
+
+int orc_crs_de_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                     const int32_t* nbr_off, const int32_t* nbr, double a0d, double a1d, double b0d, double b1d,
+                     double c0d, double c1d, double taud, int pop, int ngen, double Fd, double CRd, uint64_t seed,
+                     int cdp_begin, int cdp_end, float* pa, float* pb, float* pc, float* coher, float* stack) {
+  if (cdp_begin < 0 || cdp_end > ncdp || cdp_begin >= cdp_end) return 1;
   de_work W;
   if (de_open(&W, ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0d, a1d, b0d, b1d, c0d, c1d, taud, pop, ngen,
               Fd, CRd, seed))
     return 1;
-  for (int cdp = 0; cdp < ncdp; cdp++) {
+  for (int cdp = cdp_begin; cdp < cdp_end; cdp++) {
     de_load_cdp(&W, cdp, cdp_off, nbr_off, nbr);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int t0 = 0; t0 < ns; t0++) {
       float bx[3], bf, bs;
       de_point(&W.S, cdp, t0, bx, &bf, &bs, NULL);
-      const size_t o = (size_t)cdp * ns + t0;
+      const size_t o = (size_t)(cdp - cdp_begin) * ns + t0;
       pa[o] = bx[0];
       pb[o] = bx[1];
       pc[o] = (float)((double)bx[2] / FACTOR);

@@ -33,6 +33,13 @@ int orc_crs(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom*
             double c0, double c1, int nc, double tau, int32_t* index, float* pa, float* pb, float* pc,
             float* coher, float* stack, float* panel_sem, float* panel_stk);
 
+/* orc_crs over CDPs [cdp_begin, cdp_end) only (their supergathers may reach
+ * any CDP of the gather); outputs (and panels) have cdp_end - cdp_begin rows. */
+int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                  const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1,
+                  int nb, double c0, double c1, int nc, double tau, int cdp_begin, int cdp_end, int32_t* index,
+                  float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem, float* panel_stk);
+
 /* FP64 oracle for error statements: the CMP search with every quantity in
  * double (geometry from the integer coordinates, grid, traveltime, sums).
  * Outputs as orc_cmp but in double. */
@@ -52,6 +59,13 @@ int orc_crs_de(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_ge
                const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, double b0, double b1, double c0,
                double c1, double tau, int pop, int ngen, double F, double CR, uint64_t seed, float* pa, float* pb,
                float* pc, float* coher, float* stack);
+
+/* orc_crs_de over CDPs [cdp_begin, cdp_end) only; outputs have
+ * cdp_end - cdp_begin rows. */
+int orc_crs_de_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                     const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, double b0, double b1,
+                     double c0, double c1, double tau, int pop, int ngen, double F, double CR, uint64_t seed,
+                     int cdp_begin, int cdp_end, float* pa, float* pb, float* pc, float* coher, float* stack);
 
 /* The same search at one (cdp, t0), with its whole trajectory: pop_x[g]
  * [pop][3] (c scaled by 1e6) and pop_f[g][pop] = the population and its

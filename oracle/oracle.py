@@ -30,6 +30,9 @@ def lib():
         d, i = C.c_double, C.c_int
         L.orc_cmp.argtypes = [i, P, i, i, P, P, d, d, i, d, P, P, P, P, P, P]
         L.orc_crs.argtypes = [i, P, i, i, P, P, P, P, d, d, i, d, d, i, d, d, i, d, P, P, P, P, P, P, P, P]
+        L.orc_crs_range.argtypes = L.orc_crs.argtypes[:18] + [i, i] + L.orc_crs.argtypes[18:]
+        L.orc_crs_de_range.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, d, d, d, i, i, d, d, C.c_uint64, i, i,
+                                       P, P, P, P, P]
         L.orc_cmp_f64.argtypes = [i, P, i, i, P, P, d, d, i, d, P, P, P, P]
         L.orc_crs_eval_f64.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, i, P, P]
         L.orc_crs_de.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, d, d, d, i, i, d, d, C.c_uint64, P, P, P, P, P]
@@ -69,18 +72,22 @@ def cmp(cdp_off, geom, samples, dt_us, c0, c1, nc, tau, panel=False):
     return out
 
 
-def crs(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0, c1, nc, tau, panel=False):
+def crs(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0, c1, nc, tau, panel=False,
+        cdp_range=None):
+    """orc_crs; cdp_range = (begin, end) searches only those CDPs (rows of the
+    outputs), their supergathers reaching any CDP of the gather."""
     cdp_off, samples, geom = _c(cdp_off, np.int32), _c(samples, np.float32), _geom(geom)
     nbr_off, nbr = _c(nbr_off, np.int32), _c(nbr if len(nbr) else [0], np.int32)
     ncdp, ns = len(cdp_off) - 1, samples.shape[1]
+    cb, ce = (0, ncdp) if cdp_range is None else cdp_range
     npar = na * nb * nc
-    out = {k: np.zeros((ncdp, ns), np.int32 if k == "index" else np.float32)
+    out = {k: np.zeros((ce - cb, ns), np.int32 if k == "index" else np.float32)
            for k in ("index", "a", "b", "c", "coher", "stack")}
-    ps = np.zeros((ncdp, ns, npar), np.float32) if panel else None
-    pk = np.zeros((ncdp, ns, npar), np.float32) if panel else None
-    rc = lib().orc_crs(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, na, b0, b1,
-                       nb, c0, c1, nc, tau, _p(out["index"]), _p(out["a"]), _p(out["b"]), _p(out["c"]),
-                       _p(out["coher"]), _p(out["stack"]), _p(ps), _p(pk))
+    ps = np.zeros((ce - cb, ns, npar), np.float32) if panel else None
+    pk = np.zeros((ce - cb, ns, npar), np.float32) if panel else None
+    rc = lib().orc_crs_range(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, na,
+                             b0, b1, nb, c0, c1, nc, tau, cb, ce, _p(out["index"]), _p(out["a"]), _p(out["b"]),
+                             _p(out["c"]), _p(out["coher"]), _p(out["stack"]), _p(ps), _p(pk))
     assert rc == 0
     if panel:
         out["panel_sem"], out["panel_stk"] = ps, pk
@@ -112,14 +119,16 @@ def crs_eval_f64(cdp, cdp_off, geom, samples, dt_us, nbr_off, nbr, a, b, c, tau,
     return sem.value, stk.value
 
 
-def crs_de(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, b0, b1, c0, c1, tau, pop, ngen, F, CR, seed):
+def crs_de(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, b0, b1, c0, c1, tau, pop, ngen, F, CR, seed,
+           cdp_range=None):
     cdp_off, samples, geom = _c(cdp_off, np.int32), _c(samples, np.float32), _geom(geom)
     nbr_off, nbr = _c(nbr_off, np.int32), _c(nbr if len(nbr) else [0], np.int32)
     ncdp, ns = len(cdp_off) - 1, samples.shape[1]
-    out = {k: np.zeros((ncdp, ns), np.float32) for k in ("a", "b", "c", "coher", "stack")}
-    rc = lib().orc_crs_de(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, b0, b1,
-                          c0, c1, tau, pop, ngen, F, CR, seed, _p(out["a"]), _p(out["b"]), _p(out["c"]),
-                          _p(out["coher"]), _p(out["stack"]))
+    cb, ce = (0, ncdp) if cdp_range is None else cdp_range
+    out = {k: np.zeros((ce - cb, ns), np.float32) for k in ("a", "b", "c", "coher", "stack")}
+    rc = lib().orc_crs_de_range(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, b0,
+                                b1, c0, c1, tau, pop, ngen, F, CR, seed, cb, ce, _p(out["a"]), _p(out["b"]),
+                                _p(out["c"]), _p(out["coher"]), _p(out["stack"]))
     assert rc == 0
     return out
 
