@@ -9,7 +9,8 @@ Tolerances (written here, checked below):
   FP16 store: semblance at the reported (t0, c) within 2e-3 relative of the
       FP64 oracle (semblances >= 0.05), stack within 2e-3 of the CDP's
       amplitude scale (max |stack| of the FP64 oracle).
-  BF16 store: measured and printed; asserted only to 2e-2 (8-bit mantissa).
+  BF16 store: semblance within 5e-3 relative (8-bit mantissa: the FP16 bar
+      of 2e-3 does not hold for it), stack within 2e-3.
 """
 import numpy as np
 import pytest
@@ -28,10 +29,21 @@ def subset(cdp_off, geom, samples, cdps):
     return off, geom[rows], samples[rows]
 
 
-def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx):
+@pytest.fixture(scope="module")
+def cmp_line(sb):
+    """The CMP production line: 2000 CDPs, fold 30-120, ns 1501, dt 2 ms."""
+    return sb.synth(0, 2000, (30, 120), 1501, 2000, SEED + 1, h_max=1500.0, h_jitter=5.0, nevents=8, snr=5.0)
+
+
+def _twelve_cdps(cdp_off):
+    folds = np.diff(cdp_off)
+    return sorted(set([int(np.argmax(folds)), int(np.argmin(folds)), 0, 1999] +
+                      np.random.default_rng(5).choice(2000, 8, replace=False).tolist()))
+
+
+def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx, cmp_line):
     print("case: cmp 2000 CDPs fold 30-120 ns 1501 nc 256 fast f32", flush=True)
-    cdp_off, geom, samples = sb.synth(0, 2000, (30, 120), 1501, 2000, SEED + 1, h_max=1500.0, h_jitter=5.0,
-                                      nevents=8, snr=5.0)
+    cdp_off, geom, samples = cmp_line
     g = sb.Gather(gpu_ctx, cdp_off, geom, samples, 2000)
     out = sb.search(gpu_ctx, g, sb.cmp_params(C0, C1, 256, 0.002, sb.MODE_FAST))
     g.close()
@@ -46,9 +58,7 @@ def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx):
     want = np.where(idx < 0, np.float32(0),
                     ((c0 + inc * idx.astype(np.float32)).astype(np.float64) / 1e6).astype(np.float32))
     assert np.array_equal(out["c"], want)
-    folds = np.diff(cdp_off)
-    pick = sorted(set([int(np.argmax(folds)), int(np.argmin(folds)), 0, 1999] +
-                      np.random.default_rng(5).choice(2000, 8, replace=False).tolist()))
+    pick = _twelve_cdps(cdp_off)
     off, gs, ss = subset(cdp_off, geom, samples, pick)
     ref = orc.cmp(off, gs, ss, 2000, C0, C1, 256, 0.002, panel=True)
     ref64 = orc.cmp_f64(off, gs, ss, 2000, C0, C1, 256, 0.002, panel=True)
@@ -57,7 +67,38 @@ def test_cmp_production_size_against_oracle(sb, orc, gpu_ctx):
     assert ndiff <= 1e-3 * mine["index"].size
 
 
-@pytest.mark.parametrize("store,tol_sem,tol_stk", [("f16", 2e-3, 2e-3), ("bf16", 2e-2, 2e-2)])
+@pytest.mark.parametrize("store,tol_sem,tol_stk", [("f16", 2e-3, 2e-3), ("bf16", 5e-3, 2e-3)])
+def test_production_line_16bit_store_error_vs_fp64(sb, orc, gpu_ctx, cmp_line, store, tol_sem, tol_stk):
+    """The 2000-CDP production line stored in FP16 / BF16: on 12 CDPs at full
+    fold and ns, the reported semblance within tol_sem (relative, semblance >=
+    0.05) of the FP64 oracle's semblance at the reported (t0, c), the stack
+    within tol_stk of the CDP's amplitude where the FP64 arg-max agrees.  FP16
+    meets the 2e-3 bar; BF16 (8-bit mantissa) is held to 5e-3."""
+    print("case: cmp 2000 CDPs fold 30-120 ns 1501 nc 256 store", store, flush=True)
+    cdp_off, geom, samples = cmp_line
+    g = sb.Gather(gpu_ctx, cdp_off, geom, samples, 2000, store=sb.STORES[store])
+    out = sb.search(gpu_ctx, g, sb.cmp_params(C0, C1, 256, 0.002, sb.MODE_FAST))
+    g.close()
+    pick = _twelve_cdps(cdp_off)
+    off, gs, ss = subset(cdp_off, geom, samples, pick)
+    ref = orc.cmp_f64(off, gs, ss, 2000, C0, C1, 256, 0.002, panel=True)
+    idx, coh, stk = out["index"][pick], out["coher"][pick], out["stack"][pick]
+    ok = idx >= 0
+    sem64 = np.take_along_axis(ref["panel_sem"], np.maximum(idx, 0)[..., None], 2)[..., 0]
+    big = ok & (sem64 >= 0.05)
+    rel = np.abs(coh[big] - sem64[big]) / sem64[big]
+    same = ok & (idx == ref["index"])
+    scale = np.max(np.abs(ref["stack"]), axis=1, keepdims=True)
+    stk_err = np.abs(stk - ref["stack"]) / scale
+    print("production line store %s: semblance rel err vs FP64 max %.3g p99 %.3g (n=%d); stack err max %.3g; "
+          "index agreement with FP64 %.5f" % (store, rel.max(), np.percentile(rel, 99), big.sum(),
+                                              stk_err[same].max(), (idx == ref["index"]).mean()), flush=True)
+    assert big.sum() > 1000
+    assert rel.max() <= tol_sem and stk_err[same].max() <= tol_stk
+    assert (idx == ref["index"]).mean() > 0.99
+
+
+@pytest.mark.parametrize("store,tol_sem,tol_stk", [("f16", 2e-3, 2e-3), ("bf16", 5e-3, 2e-3)])
 def test_reduced_precision_store_error_vs_fp64(sb, orc, gpu_ctx, store, tol_sem, tol_stk):
     print("case: cmp 200 CDPs fold 60 ns 1001 nc 101 store", store, flush=True)
     cdp_off, geom, samples = sb.synth(0, 200, 60, 1001, 2000, SEED, h_max=1500.0, regular_offsets=True,
@@ -187,7 +228,7 @@ def _crs_errors(orc, out, line, cells):
     return np.array(rel), np.array(stk)
 
 
-@pytest.mark.parametrize("store,tol", [("f16", 2e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("store,tol", [("f16", 2e-3), ("bf16", 5e-3)])
 def test_crs_reduced_precision_store_error_vs_fp64(sb, orc, gpu_ctx, store, tol):
     """CRS grid search on a 16-bit store: the reported semblance and stack at
     the reported (A, B, C) against the FP64 evaluation of the same point
