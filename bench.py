@@ -239,12 +239,37 @@ def crs_workload(sb, cfg, seed):
     return cdp_off, geom, samples, nbr_off, nbr
 
 
-def crs_cpu_baseline(sb, cdp_off, geom, samples, nbr_off, nbr, ncdp_sample):
-    """The reference crs-omp2 on the first ncdp_sample CDPs of the line (their
+def crs_cpu_baseline(sb, orc, cdp_off, geom, samples, nbr_off, nbr, every=50, block=6):
+    """CPU rate of the CRS grid search on all host cores.  Headline: the C
+    restatement of the reference's double-precision search (kind "port") on
+    every `every`-th CDP of the line with its full supergather (a stratified
+    sample: line edges and interior alike).  Beside it, the reference crs-omp2
+    on the first `block` CDPs written as an SU file (their supergathers
+    clipped at that range), its own INFO line."""
+    cores = len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    b = BOX
+    sel = list(range(0, len(cdp_off) - 1, every))
+    sizes = [sum(int(cdp_off[j + 1] - cdp_off[j]) for j in nbr[nbr_off[i]:nbr_off[i + 1]]) for i in sel]
+    tuples = float(sum(sizes)) * CRS_CFG["ns"] * CRS_CFG["n"] ** 3
+    t = time.perf_counter()
+    for i in sel:
+        orc.crs(cdp_off, geom, samples, CRS_CFG["dt_us"], nbr_off, nbr, b["a0"], b["a1"], 11, b["b0"], b["b1"], 11,
+                b["c0"], b["c1"], 11, CRS_CFG["tau"], cdp_range=(i, i + 1))
+    wall = time.perf_counter() - t
+    out = dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port",
+               sample="every %dth CDP (%d CDPs, supergathers of %d-%d traces, %.3g evals) at full ns and grid, the "
+                      "oracle's double-precision restatement of crs-omp2" % (every, len(sel), min(sizes), max(sizes),
+                                                                             tuples))
+    out["reference_block"] = _crs_reference_block(sb, cdp_off, geom, samples, block)
+    return out
+
+
+def _crs_reference_block(sb, cdp_off, geom, samples, n):
+    """The reference crs-omp2 on the first n CDPs of the line (their
     supergathers restricted to that range), all host cores, its INFO line."""
     ref = os.path.join(ROOT, "oracle", "_ref", "crs-omp2")
     cores = len(os.sched_getaffinity(0))
-    n = ncdp_sample
     rows = np.arange(cdp_off[0], cdp_off[n])
     sample = "first %d CDPs (%d traces), full ns and grid" % (n, len(rows))
     if not os.path.exists(ref):
@@ -316,7 +341,8 @@ def side_configs(sb, args):
                                PEAK_FP32_VECTOR_TFLOPS)
         g.close()
     if not args.no_cpu:
-        out["crs_grid"]["cpu_baseline"] = crs_cpu_baseline(sb, cdp_off, geom, samples, nbr_off, nbr, 6)
+        print("bench: CRS CPU baseline", flush=True)
+        out["crs_grid"]["cpu_baseline"] = crs_cpu_baseline(sb, orc, cdp_off, geom, samples, nbr_off, nbr)
     del w, cdp_off, geom, samples
     print("bench: generating CRS-DE workload", flush=True)
     cdp_off, geom, samples, nbr_off, nbr = crs_workload(sb, DE_CFG, SEED + 3)
@@ -344,22 +370,34 @@ def side_configs(sb, args):
     return out
 
 
-def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, store=0):
-    """Serving loop under load.  A CMP line (96 CDPs of fold 24-1000, kept at
-    ns 501/1001/1501/2001/3001) and a CRS line (32 CDPs, ns 501/1001) are made
-    resident on the server once; requests name a line and a CDP range.
-    Poisson arrivals of CMP (70 %) and CRS (30 %) panel requests plus a probe
-    stream of 1-CDP CMP panels (ns 1501, nc 256); latency = server done -
-    submit time.  CMP requests: 1-64 CDPs, nc 21-128.  CRS requests: 1-8 CDPs,
-    5-11 values per parameter (the 21-128 per parameter of the full
-    specification offers ~10^3 times one GPU's capacity at 60 CRS
-    requests/s)."""
-    rng = np.random.default_rng(seed)
+def serving_lines(sb, seed=SEED + 4):
+    """The resident lines of the serving workload: a CMP line (96 CDPs of fold
+    24-1000, ns 3001, cut to ns 501/1001/1501/2001/3001) and a CRS line (32
+    CDPs, ns 1001, cut to 501/1001)."""
     print("bench: generating serving lines", flush=True)
     cmp_off, cmp_geom, cmp_s = sb.synth(0, 96, (24, 1000), 3001, 2000, seed, h_max=1500.0, h_jitter=5.0, nevents=8)
     crs_off, crs_geom, crs_s = sb.synth(1, 32, (24, 1000), 1001, 4000, seed + 1, h_max=800.0, mid_half_width=50.0,
                                         nevents=6)
     crs_no, crs_nb = sb.neighbors(crs_geom[crs_off[:-1]], 50)
+    lines = {}
+    for n in (501, 1001, 1501, 2001, 3001):
+        lines[("cmp", n)] = (cmp_off, cmp_geom, np.ascontiguousarray(cmp_s[:, :n]), 2000, {})
+    for n in (501, 1001):
+        lines[("crs", n)] = (crs_off, crs_geom, np.ascontiguousarray(crs_s[:, :n]), 4000,
+                             dict(nbr_off=crs_no, nbr=crs_nb))
+    return lines
+
+
+def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, store=0):
+    """Serving loop under load on the resident lines of serving_lines():
+    requests name a line and a CDP range.  Poisson arrivals at `rate` of CMP
+    (70 %) and CRS (30 %) panel requests plus a probe stream of 1-CDP CMP
+    panels (ns 1501, nc 256) at `probe_rate`; latency = server done - submit
+    time.  CMP requests: 1-64 CDPs, nc 21-128.  CRS requests: 1-8 CDPs, 5-11
+    values per parameter (the 21-128 per parameter of the full
+    specification offers ~10^3 times one GPU's capacity at 60 CRS
+    requests/s)."""
+    rng = np.random.default_rng(seed)
     b = BOX
     events = []
     t = 0.0
@@ -384,18 +422,13 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
     events.sort(key=lambda e: e[0])
     tickets = []
     with sb.Server(0, max_batch=64) as srv:
-        gid = {}
-        for n in (501, 1001, 1501, 2001, 3001):
-            gid[("cmp", n)] = srv.add_gather(cmp_off, cmp_geom, np.ascontiguousarray(cmp_s[:, :n]), 2000,
-                                             store=store)
-        for n in (501, 1001):
-            gid[("crs", n)] = srv.add_gather(crs_off, crs_geom, np.ascontiguousarray(crs_s[:, :n]), 4000,
-                                             store=store, nbr_off=crs_no, nbr=crs_nb)
+        gid = {key: srv.add_gather(l[0], l[1], l[2], l[3], store=store, **l[4]) for key, l in lines.items()}
         resident = srv.stats()["hbm_reserved"]
         # warm-up: one request of each kind
         for ev in events[:4]:
             srv.wait(srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5]))
-        print("bench: serving %d requests over %.1f s" % (len(events), seconds), flush=True)
+        print("bench: serving %d requests over %.1f s (%.0f + %.0f req/s offered)" %
+              (len(events), seconds, rate, probe_rate), flush=True)
         start = time.perf_counter()
         for ev in events:
             d = ev[0] - (time.perf_counter() - start)
@@ -404,16 +437,21 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
             tickets.append((ev[1], srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5])))
         lat = {"cmp": [], "crs": [], "probe": []}
         wait = {"cmp": [], "crs": [], "probe": []}
+        failed = 0
+        last_done = 0.0
         for kind, tk in tickets:
             out, st = srv.wait(tk)
             if out is not None:
                 lat[kind].append(st["t_done_ms"] - st["t_submit_ms"])
                 wait[kind].append(st["t_admit_ms"] - st["t_submit_ms"])
+            else:
+                failed += 1
         wall = time.perf_counter() - start
         stats = srv.stats()
     res = dict(store="f32" if store == 0 else "f16", offered_rps=rate + probe_rate, duration_s=round(wall, 3),
-               served=stats["done"] - 4,
-               launches=stats["launches"] - 4, resident_gb=round(resident / 1e9, 2))
+               requests=len(events), served=stats["done"] - 4, failed=failed,
+               served_rps=round((stats["done"] - 4) / wall, 1), launches=stats["launches"] - 4,
+               resident_gb=round(resident / 1e9, 2))
     for k, v in lat.items():
         v = np.array(v)
         if len(v):
@@ -422,6 +460,26 @@ def serving_bench(sb, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, 
                           max_ms=float(v.max()), queue_p50_ms=float(np.percentile(w, 50)),
                           queue_p99_ms=float(np.percentile(w, 99)))
     return res
+
+
+def serving_sweep(sb, lines, rates=(200, 500, 1000, 2000), seconds=2.0):
+    """The serving workload over its arrival-rate range, both stores: per
+    offered rate, served vs offered, p50/p99 per request kind, and the
+    lowest offered rate at which the 1-CDP probe's p99 exceeds 50 ms."""
+    out = {}
+    for store, name in ((0, "f32"), (1, "f16")):
+        curve = []
+        for r in rates:
+            res = serving_bench(sb, lines, seconds=seconds, rate=float(r), store=store)
+            curve.append(dict(offered_rps=res["offered_rps"], requests=res["requests"], served=res["served"],
+                              failed=res["failed"], served_rps=res["served_rps"], duration_s=res["duration_s"],
+                              **{k: {q: round(res[k][q], 2) for q in ("p50_ms", "p99_ms", "queue_p99_ms")}
+                                 for k in ("probe", "cmp", "crs") if k in res}))
+        crossing = next((c["offered_rps"] for c in curve if c.get("probe", {}).get("p99_ms", 0) > 50.0), None)
+        out[name] = dict(curve=curve, probe_p99_over_50ms_from_rps=crossing)
+    out["what"] = ("offered = mixed CMP/CRS panel requests at the named rate + 100 req/s of 1-CDP CMP probes, %.0f s "
+                   "of arrivals per point; latency = done - submit" % seconds)
+    return out
 
 
 def launch_modes(sb, ctx, g, prm, steps=3):
@@ -484,6 +542,7 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="skip the CRS grid and CRS-DE side measurements")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop load test")
     ap.add_argument("--no-modes", action="store_true", help="skip the persistent / LDS / graph launch-mode timings")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep over arrival rates")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -529,8 +588,12 @@ def main():
     skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
     lds = bool(f32["lds_staged"])
     kernel = "k_search_lds" if lds else "k_search_fast"
+    # HBM bytes per launch of the timed shape, from rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes over the same workload and launch shape (null when
+    # that shape was not profiled)
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s_cmp_2000cdp.json" % kernel)
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_cmp_2000cdp_v%d_p%d.json" % (kernel, f32["tune"][0],
+                                                                                f32["tune"][1]))
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -556,16 +619,18 @@ def main():
             "launch": {"variant": f16["tune"][0], "pchunk": f16["tune"][1]}},
         "launch_modes_ms": {k: results[k].get("modes") for k in results},
         "panel_latency_ms": {"p50": p50, "p99": p99, "what": "1-CDP CMP panel, upload+search+download, sequential"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_VECTOR_TFLOPS, "traffic": traffic,
+        "value_evaluated": f32["value"] * (1.0 - skip),
+        "roofline": {"bound": "valu", "achieved": achieved * (1.0 - skip), "peak": PEAK_FP32_VECTOR_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved * (1.0 - skip) / PEAK_FP32_VECTOR_TFLOPS, "traffic": traffic,
                      "kernel": kernel + " (CMP, W=3)", "kernel_ms": f32["search_ms"],
-                     "flop_per_eval": FLOP_PER_TUPLE_CMP,
-                     "skipped_frac": skip, "achieved_evaluated": achieved * (1.0 - skip),
-                     "frac_evaluated": achieved * (1.0 - skip) / PEAK_FP32_VECTOR_TFLOPS,
-                     "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s "
-                             "(packed FP32; 78.6 TFLOP/s unpacked); achieved counts the reference's evaluations, "
-                             "skipped_frac of which have no result (two invalid traces) and are never run -- "
-                             "achieved_evaluated counts only those run; " + (
+                     "flop_per_eval": FLOP_PER_TUPLE_CMP, "skipped_frac": skip,
+                     "achieved_reference_count": achieved,
+                     "frac_reference_count": achieved / PEAK_FP32_VECTOR_TFLOPS,
+                     "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s; "
+                             "achieved counts the evaluations the kernel runs: the reference's count minus "
+                             "skipped_frac of it, (t0, c) cells that have two invalid traces and no result in any "
+                             "arithmetic (achieved_reference_count counts all of them, as the reference's rate "
+                             "does); value is the reference's count per second; " + (
                                  "windows come from LDS (two ds_read2_b32 per evaluation; a workgroup stages the "
                                  "ranges of as many of its traces as fit, once, and reads the rest from global "
                                  "memory): bound by VALU issue and the LDS data path, the staging latency partly "
@@ -582,10 +647,14 @@ def main():
         print("bench: config-0 line through the CLI", flush=True)
         line["cli_config0"] = cli_config0(sb)
     if not args.no_serving:
-        line["serving"] = serving_bench(sb)
-        line["serving_fp16_store"] = serving_bench(sb, store=sb.STORE_F16)
+        lines = serving_lines(sb)
+        line["serving"] = serving_bench(sb, lines)
+        line["serving_fp16_store"] = serving_bench(sb, lines, store=sb.STORE_F16)
         if "probe" in line["serving"]:
             line["panel_latency_ms"]["p99_under_load"] = line["serving"]["probe"]["p99_ms"]
+        if not args.no_sweep:
+            line["serving_sweep"] = serving_sweep(sb, lines)
+        del lines
     print(json.dumps(line), flush=True)
 
 

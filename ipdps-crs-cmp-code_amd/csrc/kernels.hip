@@ -696,51 +696,32 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
           }
         }
       }
-      constexpr bool HALF = STORE == SB_STORE_F16 && W <= 7;
       Win<W + 1> sv[R][CC];
-      WinH<W + 1> sh[R][CC];
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int ci = 0; ci < CC; ci++) {
-          if constexpr (HALF)
-            sh[r][ci] = load_win_h<W + 1>(rs, off[r][ci], shf[r][ci]);
-          else
-            sv[r][ci] = load_win<W + 1, STORE>(rs, off[r][ci], shf[r][ci]);
-        }
+        for (int ci = 0; ci < CC; ci++) sv[r][ci] = load_win<W + 1, STORE>(rs, off[r][ci], shf[r][ci]);
+      // 16-bit stores: samples restored exactly (stored value times the
+      // trace's power-of-two scale), then the FP32 evaluation for every store
+      // -- the LDS-staged search restores them once while staging.
+      if constexpr (STORE != SB_STORE_F32) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int ci = 0; ci < CC; ci++)
+#pragma unroll
+            for (int j = 0; j <= W; j++) sv[r][ci].v[j] *= s.scale;
+      }
 #pragma unroll
       for (int r = 0; r < R; r++) {
 #pragma unroll
         for (int ci = 0; ci < CC; ci++) {
           const float x = xx[r][ci];
-          if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll
-            for (int j = 0; j < W; j++) {
-              const float v = fmaf(__fsub_rn(sv[r][ci].v[j + 1], sv[r][ci].v[j]), x, sv[r][ci].v[j]);
-              num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
-              den[r][ci] = fmaf(v, v, den[r][ci]);
-            }
-          } else if constexpr (HALF) {
-            // v = s_j + x s_{j+1} - x s_j: two mixed-precision FMAs
-            // (v_fma_mix_f32) per sample, halves read in place.
-            float q = 0.f;
-#pragma unroll
-            for (int j = 0; j < W; j++) {
-              const float hj = (float)sh[r][ci].h[j], hj1 = (float)sh[r][ci].h[j + 1];
-              const float v = fmaf(-x, hj, fmaf(x, hj1, hj));
-              num[r][ci][j] = fmaf(s.scale, v, num[r][ci][j]);
-              q = fmaf(v, v, q);
-            }
-            den[r][ci] = fmaf(s.scale2, q, den[r][ci]);
-          } else {
-            float q = 0.f;
-#pragma unroll
-            for (int j = 0; j < W; j++) {
-              const float v = fmaf(__fsub_rn(sv[r][ci].v[j + 1], sv[r][ci].v[j]), x, sv[r][ci].v[j]);
-              num[r][ci][j] = fmaf(s.scale, v, num[r][ci][j]);
-              q = fmaf(v, v, q);
-            }
-            den[r][ci] = fmaf(s.scale2, q, den[r][ci]);
+          for (int j = 0; j < W; j++) {
+            const float v = fmaf(__fsub_rn(sv[r][ci].v[j + 1], sv[r][ci].v[j]), x, sv[r][ci].v[j]);
+            num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
+            den[r][ci] = fmaf(v, v, den[r][ci]);
           }
         }
       }

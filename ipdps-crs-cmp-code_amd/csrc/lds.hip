@@ -15,8 +15,8 @@
 // bound it, widened by 2 for the approximate square root.  The ranges of the
 // first K traces in slot order (all of them when they fit; the nearest
 // offsets have the shortest ranges) are staged once as FP32 in 16-byte quads
-// aligned as in the row (the stored bits converted exactly: 16-bit stores
-// keep their per-trace scale for the arithmetic), and every wave then takes
+// aligned as in the row (16-bit stores restored exactly: the stored value
+// times the trace's power-of-two scale), and every wave then takes
 // groups of CC parameters in turn, reading the windows of those traces at
 // byte address base_k + 4 floor(t) (one v_lshl_add_u32; gated windows read
 // the zeros at address 0) and the windows of the other traces from global
@@ -24,7 +24,9 @@
 //
 // Every rounding step matches k_search_fast, in the same trace order, with
 // the same general/prefix split, last-trace gate and second-to-last-trace
-// validity, so both kernels produce identical keys.
+// validity, so both kernels produce identical keys.  Every store runs the
+// FP32 evaluation on its restored samples, so a trace's contribution is the
+// same whether its window comes from LDS or from global memory.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -64,11 +66,12 @@ __device__ __forceinline__ T lds_at(unsigned addr) {
 }
 
 // One trace of the CDP as the trace loop uses it: h and the 16-bit stores'
-// scales from the slot table (scalar loads), the window base from LDS.
+// scale (restores windows read from global memory) from the slot table
+// (scalar loads), the window base from LDS.
 struct TraceRec {
   float h;
   unsigned base;  // the window of traveltime t starts at LDS byte base + 4 floor(t)
-  float scale, scale2;
+  float scale;
 };
 
 // A trace's entry in the LDS table (h for the range scan, then the base).
@@ -228,13 +231,13 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           if constexpr (GATE != 1) off = ok ? off : kOob;
           const Win<W + 1> wv = load_win<W + 1, STORE>(rs, off, shf);
 #pragma unroll
-          for (int j = 0; j <= W; j++) ev[r][ci].v[j] = wv.v[j];
+          for (int j = 0; j <= W; j++) ev[r][ci].v[j] = STORE == SB_STORE_F32 ? wv.v[j] : wv.v[j] * t.scale;
         }
       }
     }
   };
   // k_search_fast's window arithmetic, operation for operation.
-  auto consume = [&](const TraceRec& t, const Ev<W>(&ev)[R][CC], auto gate_tag) {
+  auto consume = [&](const Ev<W>(&ev)[R][CC], auto gate_tag) {
     constexpr int GATE = decltype(gate_tag)::value;
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -243,31 +246,11 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         const Ev<W>& e = ev[r][ci];
         if constexpr (GATE != 1) cnt[r][ci] += e.ok ? 1 : 0;
         const float x = e.x;
-        if constexpr (STORE == SB_STORE_F32) {
 #pragma unroll
-          for (int j = 0; j < W; j++) {
-            const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
-            num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
-            den[r][ci] = fmaf(v, v, den[r][ci]);
-          }
-        } else if constexpr (STORE == SB_STORE_F16) {
-          float q = 0.f;
-#pragma unroll
-          for (int j = 0; j < W; j++) {
-            const float v = fmaf(-x, e.v[j], fmaf(x, e.v[j + 1], e.v[j]));
-            num[r][ci][j] = fmaf(t.scale, v, num[r][ci][j]);
-            q = fmaf(v, v, q);
-          }
-          den[r][ci] = fmaf(t.scale2, q, den[r][ci]);
-        } else {
-          float q = 0.f;
-#pragma unroll
-          for (int j = 0; j < W; j++) {
-            const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
-            num[r][ci][j] = fmaf(t.scale, v, num[r][ci][j]);
-            q = fmaf(v, v, q);
-          }
-          den[r][ci] = fmaf(t.scale2, q, den[r][ci]);
+        for (int j = 0; j < W; j++) {
+          const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
+          num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
+          den[r][ci] = fmaf(v, v, den[r][ci]);
         }
       }
     }
@@ -282,12 +265,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
     TraceRec t;
     t.h = slc[k].h;
     t.base = with_base ? lds_at<unsigned>((unsigned)(L::kTr + 8 * k + 4)) : 0u;
-    if constexpr (STORE == SB_STORE_F32) {
-      t.scale = t.scale2 = 1.f;
-    } else {
-      t.scale = slc[k].scale;
-      t.scale2 = slc[k].scale2;
-    }
+    t.scale = STORE == SB_STORE_F32 || with_base ? 1.f : slc[k].scale;
     return t;
   };
   // Traces [kb, ke): each trace's windows are read and accumulated in turn
@@ -299,7 +277,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       const TraceRec t = tr(k, FROM_LDS);
       Ev<W> A[R][CC];
       issue(t, FROM_LDS ? 0 : slc[k].row, A, gate_tag, lds_tag);
-      consume(t, A, gate_tag);
+      consume(A, gate_tag);
     }
   };
   // All traces of the group, in slot order: traces [0, K) from their staged
@@ -428,6 +406,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         __amdgpu_buffer_rsrc_t rs[G];
         int qlo[G], nq[G];
         unsigned dq[G];
+        float sc[G];  // 16-bit stores: the trace's scale (restored exactly: a power of two)
         int nqm = 0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -439,6 +418,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           dq[g] = (unsigned)(smp + 16 * qb);
           if (lane == 0 && kg + g < K) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
           const int row = slc[k].row;
+          sc[g] = STORE == SB_STORE_F32 ? 1.f : slc[k].scale;
           rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
                                                     (short)0, (int)(a.stride * ESZ), kRsrcWord3);
         }
@@ -457,10 +437,10 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
                 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
                 const h2v lo2 = __builtin_bit_cast(h2v, d0);
                 const h2v hi2 = __builtin_bit_cast(h2v, d1);
-                v[g] = f4{(float)lo2.x, (float)lo2.y, (float)hi2.x, (float)hi2.y};
+                v[g] = f4{(float)lo2.x * sc[g], (float)lo2.y * sc[g], (float)hi2.x * sc[g], (float)hi2.y * sc[g]};
               } else {
-                v[g] = f4{__uint_as_float(d0 << 16), __uint_as_float(d0 & 0xffff0000u), __uint_as_float(d1 << 16),
-                          __uint_as_float(d1 & 0xffff0000u)};
+                v[g] = f4{__uint_as_float(d0 << 16) * sc[g], __uint_as_float(d0 & 0xffff0000u) * sc[g],
+                          __uint_as_float(d1 << 16) * sc[g], __uint_as_float(d1 & 0xffff0000u) * sc[g]};
               }
             }
           }
