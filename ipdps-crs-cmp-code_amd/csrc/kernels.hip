@@ -636,8 +636,17 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
     // GATE 0: full validity test and count (general path); 1: none (prefix
     // path, all but the last trace); 2: the t^2 bound only (prefix path,
     // last trace).
-    auto trace = [&](const SlotF& s, auto gate_tag) {
+    // SHAB: every parameter of the pass has the same (a, b) (c is the
+    // fastest grid index, so most passes of CC consecutive parameters do),
+    // and Q = (a, b, T0) part of the traveltime is formed once per trace.
+    auto trace = [&](const SlotF& s, auto gate_tag, auto shab_tag) {
       constexpr int GATE = decltype(gate_tag)::value;
+      constexpr bool SHAB = decltype(shab_tag)::value;
+      float Qs[R];
+      if constexpr (CRS && SHAB) {
+#pragma unroll
+        for (int r = 0; r < R; r++) Qs[r] = fmaf(K2[0], s.m2, fmaf(K1[r][0], s.m, T0sq[r]));
+      }
       // Rows are addressed from `lead` bytes before their start (the
       // allocation has kSamplesLead bytes in front of row 0): FP32 from tau
       // samples, so a window's byte offset is 4 * floor(t); 16-bit stores
@@ -657,8 +666,14 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
         for (int ci = 0; ci < CC; ci++) {
           float tt;
           if constexpr (CRS) {
-            // (T0 + a m)^2 + b m^2 + c h expanded: three FMAs per evaluation
-            tt = fmaf(K2[ci], s.m2, fmaf(K1[r][ci], s.m, fmaf(pc[ci], s.h, T0sq[r])));
+            // (T0 + a m)^2 + b m^2 + c h expanded as c h + Q(a, b), Q =
+            // (a^2 + b) m^2 + 2 T0 a m + T0^2: one FMA per evaluation when
+            // the pass shares (a, b), three otherwise -- the same roundings
+            // either way, so an evaluation's bits do not depend on the pass.
+            if constexpr (SHAB)
+              tt = fmaf(pc[ci], s.h, Qs[r]);
+            else
+              tt = fmaf(pc[ci], s.h, fmaf(K2[ci], s.m2, fmaf(K1[r][ci], s.m, T0sq[r])));
           } else {
             tt = fmaf(pc[ci], s.h, T0sq[r]);
           }
@@ -746,12 +761,21 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
       if (__builtin_amdgcn_ballot_w64(live) == 0ull) continue;
     }
     if (!prefix) {
-      SlotF nx = sl[0];
-      for (int k = 0; k < S; k++) {
-        const SlotF s = nx;
-        nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
-        trace(s, std::integral_constant<int, 0>{});
-      }
+      auto general = [&](auto shab_tag) {
+        SlotF nx = sl[0];
+        for (int k = 0; k < S; k++) {
+          const SlotF s = nx;
+          nx = sl[k + 1 < S ? k + 1 : k];  // next slot's scalar load in flight
+          trace(s, std::integral_constant<int, 0>{}, shab_tag);
+        }
+      };
+      // the pass's parameters p .. last share (a, b) iff their (ia, ib) index
+      // p / nc does (wave-uniform)
+      const int last = min(p + CC, pw1) - 1;
+      if (CRS && p / a.nc == last / a.nc)
+        general(std::true_type{});
+      else
+        general(std::false_type{});
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll
@@ -761,9 +785,9 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
       for (int k = 0; k + 1 < S; k++) {
         const SlotF s = nx;
         nx = sl[k + 1];
-        trace(s, std::integral_constant<int, 1>{});
+        trace(s, std::integral_constant<int, 1>{}, std::false_type{});
       }
-      trace(nx, std::integral_constant<int, 2>{});
+      trace(nx, std::integral_constant<int, 2>{}, std::false_type{});
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll

@@ -26,8 +26,11 @@ int lds_slab_offset(const FastVariant& v, int S);
 extern const FastVariant kFastVariants[];
 extern const int kNumFastVariants;
 // Shape of an untuned ctx: LDS-staged, 8 waves, 52 KiB (CMP; searches it
-// cannot stage run the first global-memory shape).
+// cannot stage run kGlobalFallbackVariant).
 constexpr int kDefaultFastVariant = 5;
+// Global-memory shape an LDS-staged variant falls back to (R = 1, CC = 4:
+// the fastest global shape on the CRS grid and on the CMP line).
+constexpr int kGlobalFallbackVariant = 1;
 
 bool fast_supported(int w);
 

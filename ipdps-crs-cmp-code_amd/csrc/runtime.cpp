@@ -644,11 +644,16 @@ const WorkItem* work_table(const sb_ctx* ctx) {
 }
 
 int choose_pchunk(const sb_ctx* ctx, int ncdp, int ntiles, int npar, int unit) {
-  // Enough blocks to fill every CU several times; chunks a multiple of
-  // `unit` (waves per block x parameters per wave pass).
+  // Enough blocks to fill every CU several times, and at most ~24 passes
+  // over the traces per wave (long blocks leave the CUs unevenly loaded at
+  // the end of a launch: the 11^3 CRS grid over 500 CDPs ran 438 ms in one
+  // chunk, 404 ms in four); chunks a multiple of `unit` (waves per block x
+  // parameters per wave pass).
   const int64_t want = (int64_t)ctx->num_cus * 8;
   const int64_t have = (int64_t)ncdp * ntiles;
-  int nchunks = (int)std::min<int64_t>((want + have - 1) / have, (npar + unit - 1) / unit);
+  const int64_t by_fill = (want + have - 1) / have;
+  const int64_t by_length = (npar + 24 * unit - 1) / (24 * unit);
+  int nchunks = (int)std::min<int64_t>(std::max(by_fill, by_length), (npar + unit - 1) / unit);
   nchunks = std::max(1, nchunks);
   int pchunk = (npar + nchunks - 1) / nchunks;
   pchunk = (pchunk + unit - 1) / unit * unit;
@@ -718,11 +723,12 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
       lds = true;
       a.lds_bytes = v.lds_kb * 1024;
     }
-    // The global-memory kernel runs the first global shape when the LDS
-    // slabs do not apply.  Both kernels read the same work table.
+    // The global-memory kernel runs kGlobalFallbackVariant when the LDS
+    // slabs do not apply (CRS, or CDPs too large to stage).  Both kernels
+    // read the same work table.
     if (v.lds_kb > 0 && !lds) {
-      variant = 0;
-      v = kFastVariants[0];
+      variant = kGlobalFallbackVariant;
+      v = kFastVariants[variant];
     }
     family = 0;
     ntiles = (g->ns + 64 * v.R - 1) / (64 * v.R);

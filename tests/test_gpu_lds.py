@@ -106,7 +106,7 @@ def test_lds_falls_back_when_it_cannot_stage(sb, gpu_ctx):
         prm = sb.cmp_params(1.98e-7, 1.77e-6, 16, 0.002)
         ref, _ = _search(sb, gpu_ctx, g, prm, 0)
         out, t = _search(sb, gpu_ctx, g, prm, 4)
-        assert t["lds_staged"] == 0 and t["variant"] == 0, t
+        assert t["lds_staged"] == 0 and t["variant"] == 1, t
         for k in KEYS:
             assert np.array_equal(out[k], ref[k]), k
     finally:
