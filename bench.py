@@ -585,6 +585,7 @@ def main():
     f32 = results["f32"]
     f16 = results.get("f16")
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
+    fma_rate = sb.fp32_fma_rate(0)  # the device's sustained FP32 FMA rate without packed FP32
     skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
     lds = bool(f32["lds_staged"])
     kernel = "k_search_lds" if lds else "k_search_fast"
@@ -626,7 +627,11 @@ def main():
                      "flop_per_eval": FLOP_PER_TUPLE_CMP, "skipped_frac": skip,
                      "achieved_reference_count": achieved,
                      "frac_reference_count": achieved / PEAK_FP32_VECTOR_TFLOPS,
-                     "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s; "
+                     "fp32_fma_rate_measured": fma_rate,
+                     "frac_of_measured_fma_rate": achieved * (1.0 - skip) / fma_rate,
+                     "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s, "
+                             "which counts packed FP32 -- the kernels run none, and fp32_fma_rate_measured is the "
+                             "FMA rate this device sustains without it (independent FMA chains, full grid); "
                              "achieved counts the evaluations the kernel runs: the reference's count minus "
                              "skipped_frac of it, (t0, c) cells that have two invalid traces and no result in any "
                              "arithmetic (achieved_reference_count counts all of them, as the reference's rate "

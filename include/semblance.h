@@ -291,7 +291,20 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
  * counter) instead of one workgroup per work item.  Results are the same.
  * No reference counterpart (the reference launches one grid per CDP,
  * CMP/CUDA/src/main.cu:271-304). */
-enum { SB_OPT_PERSISTENT = 1, SB_OPT_STREAM_ROWS = 2 /* traces per sb_search_streamed batch, 0 = auto */ };
+/* SB_OPT_DM_AZIMUTH: the midpoint displacement m of the CRS supergathers of
+ * gathers created afterwards on this ctx (sb_gather_create,
+ * sb_search_streamed).  -1 (default): the reference's unsigned distance
+ * m = |dm| to the CDP's first trace (CRS/OpenMP/src/main.cpp:189-198);
+ * 0..359999: the signed projection of dm onto the azimuth, in millidegrees
+ * clockwise from +y (north): m = dx sin(az) + dy cos(az), m^2 = m m, so the
+ * A term tells dips on either side apart.  An extension behind crs-ocl-de's
+ * -azimuth flag, which the reference documents but does not ship
+ * (README.md:94,115,131). */
+enum {
+  SB_OPT_PERSISTENT = 1,
+  SB_OPT_STREAM_ROWS = 2, /* traces per sb_search_streamed batch, 0 = auto */
+  SB_OPT_DM_AZIMUTH = 3
+};
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);
 
@@ -327,6 +340,14 @@ int sb_graph_create(sb_ctx* ctx, const sb_gather* g, int kind, const void* param
                     sb_result* r, sb_graph** out);
 int sb_graph_launch(sb_graph* gr);
 void sb_graph_destroy(sb_graph* gr);
+
+/* --------------------------------------------------- device measurement */
+/* The FP32 vector FMA rate the device sustains, in TFLOP/s (2 FLOP per FMA):
+ * independent FMA chains in every lane of a grid that fills every SIMD, no
+ * memory traffic.  The search kernels run no packed FP32, so this -- not the
+ * published peak, which counts packed FP32 -- is the ceiling of their
+ * arithmetic.  No reference counterpart (a measurement for the roofline). */
+int sb_device_fp32_fma_rate(int device, double* tflops);
 
 /* ------------------------------------------------------ synthetic data */
 /* Seeded synthetic gathers for the benchmark configurations (host, OpenMP).

@@ -17,6 +17,7 @@ STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 MODE_FAST, MODE_EXACT = 0, 1
 OPT_PERSISTENT = 1
 OPT_STREAM_ROWS = 2
+OPT_DM_AZIMUTH = 3
 GRAPH_CRS_DE = 2
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
 
@@ -100,7 +101,7 @@ EXPORTS = [
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
     "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
-    "sb_search_streamed", "sb_cmp_run_noalloc",
+    "sb_search_streamed", "sb_cmp_run_noalloc", "sb_device_fp32_fma_rate",
 ]
 
 
@@ -180,6 +181,7 @@ def lib():
         L.sb_graph_destroy.argtypes = [P]
         L.sb_graph_destroy.restype = None
         L.sb_cmp_run_noalloc.argtypes = [P, P, C.POINTER(CmpParams), C.c_int, C.c_int, C.POINTER(DeviceArrays)]
+        L.sb_device_fp32_fma_rate.argtypes = [C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -196,6 +198,13 @@ def _i32(a):
 
 def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def fp32_fma_rate(device=0):
+    """sb_device_fp32_fma_rate: the device's sustained FP32 FMA rate, TFLOP/s."""
+    t = C.c_double()
+    check(lib().sb_device_fp32_fma_rate(device, C.byref(t)))
+    return t.value
 
 
 # ---------------------------------------------------------------- host side

@@ -2,11 +2,18 @@
 // reference documents for crs-ocl-de (README.md:88-106).  Outputs as crs.
 //
 //   -ngen    generations (population size from SB_DE_POP, default 32;
-//            F = SB_DE_F (0.5), CR = SB_DE_CR (0.9), seed = SB_DE_SEED (1))
+//            F = SB_DE_F (0.5), CR = SB_DE_CR (0.9), seed = SB_DE_SEED (1),
+//            a decimal or 0x-hex 64-bit integer)
 //   -d       device: 1-based index as in the reference; 0 = the first device
-//   -azimuth accepted for compatibility; the traveltime uses the unsigned
-//            midpoint distance of the CRS reference and does not depend on it
-//   -na/-nb/-nc accepted for compatibility; the search is continuous
+//   -azimuth degrees clockwise from north.  By default the traveltime uses the
+//            CRS reference's unsigned midpoint distance |dm| and the value is
+//            only checked; with SB_AZIMUTH=project it uses the signed
+//            projection of dm onto that direction (SB_OPT_DM_AZIMUTH)
+//   -na/-nb/-nc required and parsed as integers, for compatibility: they
+//            size the CRS grid search, and differential evolution searches
+//            the continuous box instead of a grid
+#include <cmath>
+
 #include "cli_common.hpp"
 
 static double env_d(const char* k, double dflt) {
@@ -56,10 +63,11 @@ int main(int argc, const char** argv) {
   prm.c0 = to_d(p.get("-c0", true));
   prm.c1 = to_d(p.get("-c1", true));
   prm.tau = to_d(p.get("-tau", true));
-  (void)to_i(p.get("-na", true));
-  (void)to_i(p.get("-nb", true));
-  (void)to_i(p.get("-nc", true));
-  (void)to_d(p.get("-azimuth", true));
+  for (const char* f : {"-na", "-nb", "-nc"}) (void)to_i(p.get(f, true));
+  const double azimuth = to_d(p.get("-azimuth", true));
+  const char* az_mode = std::getenv("SB_AZIMUTH");
+  const bool project = az_mode && !std::strcmp(az_mode, "project");
+  if (az_mode && !project && std::strcmp(az_mode, "distance")) log(FAIL, "SB_AZIMUTH must be project or distance");
   prm.ngen = to_i(p.get("-ngen", true));
   const int aph = to_i(p.get("-aph", true));
   const int apm = to_i(p.get("-apm", true));
@@ -75,6 +83,10 @@ int main(int argc, const char** argv) {
   Session s;
   s.load(path, SB_METHOD_CRS, aph, apm);
   s.open(env_device(d > 0 ? d - 1 : 0));
+  if (project) {
+    const double deg = std::fmod(std::fmod(azimuth, 360.0) + 360.0, 360.0);
+    check(sb_ctx_set_option(s.ctx, SB_OPT_DM_AZIMUTH, (int)std::lround(deg * 1000.0) % 360000), "azimuth");
+  }
   const double t0 = now_s();
   HostOut out((size_t)s.ncdp * s.ns);
   check(s.search(SB_GRAPH_CRS_DE, &prm, &out.view), "CRS-DE search");

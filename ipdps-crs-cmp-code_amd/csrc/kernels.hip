@@ -33,8 +33,12 @@ namespace sb {
 // ------------------------------------------------------ per-trace geometry
 // CMP reference arithmetic (float, CMP/OpenMP/src/main.cpp:126-136) and the
 // FP64 geometry (the CRS reference arithmetic, CRS/OpenMP/src/main.cpp:164-177).
+// msign: the sign of the reference's coordinate factor (-1 for a negative
+// scalco, whose 1/scalco factor negates the midpoints; harmless for its
+// distances, undone where a signed projection needs true coordinates).
 __global__ void k_prep_traces(const TraceGeom* __restrict__ g, int ntr, float* __restrict__ h_ref32,
-                              double* __restrict__ h0, double* __restrict__ m0x, double* __restrict__ m0y) {
+                              double* __restrict__ h0, double* __restrict__ m0x, double* __restrict__ m0y,
+                              float* __restrict__ msign) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntr) return;
@@ -60,6 +64,7 @@ __global__ void k_prep_traces(const TraceGeom* __restrict__ g, int ntr, float* _
       s = 1.0 / s;
     m0x[i] = ((double)t.gx + (double)t.sx) * s * 0.5;
     m0y[i] = ((double)t.gy + (double)t.sy) * s * 0.5;
+    msign[i] = s < 0 ? -1.f : 1.f;
     double hx = ((double)t.gx - (double)t.sx) * s;
     double hy = ((double)t.gy - (double)t.sy) * s;
     double hxx = hx * hx;
@@ -112,13 +117,17 @@ __global__ __launch_bounds__(256) void k_cmp_slots(int cdp0, const int32_t* __re
 
 // CRS supergather slot tables, one block per CDP (CDPs cdp0 + blockIdx.x):
 // neighbour CDPs in list order, traces in gather order; distances to the
-// CDP's first trace (CRS/OpenMP/src/main.cpp:179-198, 287-311).
+// CDP's first trace (CRS/OpenMP/src/main.cpp:179-198, 287-311).  With az set
+// (an extension the reference only names: crs-ocl-de's -azimuth,
+// README.md:94) m is instead the signed projection of the displacement onto
+// the azimuth direction, m = dx az_s + dy az_c, and m^2 = m m.
 __global__ void k_crs_slots(int cdp0, const int32_t* __restrict__ cdp_off, const int32_t* __restrict__ nbr_off,
                             const int32_t* __restrict__ nbr, const int32_t* __restrict__ slot_off,
                             const double* __restrict__ m0x, const double* __restrict__ m0y,
                             const double* __restrict__ h0, const float* __restrict__ scale, double idt,
                             const char* samples, uint64_t row_bytes, SlotF* __restrict__ sf,
-                            SlotD* __restrict__ sd, SlotF* __restrict__ sp) {
+                            SlotD* __restrict__ sd, SlotF* __restrict__ sp, const float* __restrict__ msign, int az,
+                            double az_s, double az_c) {
 #pragma clang fp contract(off)
   const int i = cdp0 + blockIdx.x;
   const int ref = cdp_off[i];
@@ -130,10 +139,26 @@ __global__ void k_crs_slots(int cdp0, const int32_t* __restrict__ cdp_off, const
     for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
       double dx = m0x[r] - rx;
       double dy = m0y[r] - ry;
-      double dxx = dx * dx;
-      double dyy = dy * dy;
-      double m2 = dxx + dyy;
-      double m = sqrt(m2);
+      double m2, m;
+      if (az) {
+        // true coordinates: the reference's factor sign undone
+        const double sr = (double)msign[ref], st = (double)msign[r];
+        double tx = m0x[r] * st;
+        double ty = m0y[r] * st;
+        double rtx = rx * sr;
+        double rty = ry * sr;
+        dx = tx - rtx;
+        dy = ty - rty;
+        double px = dx * az_s;
+        double py = dy * az_c;
+        m = px + py;
+        m2 = m * m;
+      } else {
+        double dxx = dx * dx;
+        double dyy = dy * dy;
+        m2 = dxx + dyy;
+        m = sqrt(m2);
+      }
       double h = h0[r];
       SlotD d;
       d.m = m;
@@ -1106,9 +1131,9 @@ __global__ __launch_bounds__(256) void k_search_fast_persistent(SearchArgs a) {
 #define SB_CHECK_LAUNCH() (hipGetLastError())
 
 hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
-                              hipStream_t st) {
+                              float* msign, hipStream_t st) {
   if (ntr <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prep_traces, dim3((ntr + 255) / 256), dim3(256), 0, st, g, ntr, h_ref32, h0, m0x, m0y);
+  hipLaunchKernelGGL(k_prep_traces, dim3((ntr + 255) / 256), dim3(256), 0, st, g, ntr, h_ref32, h0, m0x, m0y, msign);
   return SB_CHECK_LAUNCH();
 }
 
@@ -1122,10 +1147,11 @@ hipError_t launch_cmp_slots(int cdp0, int ncdp, const int32_t* cdp_off, const do
 hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
                             const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
                             const float* scale, double idt, const void* samples, uint64_t row_bytes, SlotF* sf,
-                            SlotD* sd, SlotF* sp, hipStream_t st) {
+                            SlotD* sd, SlotF* sp, const float* msign, int az, double az_s, double az_c,
+                            hipStream_t st) {
   if (ncdp <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_crs_slots, dim3(ncdp), dim3(256), 0, st, cdp0, cdp_off, nbr_off, nbr, slot_off, m0x, m0y, h0,
-                     scale, idt, (const char*)samples, row_bytes, sf, sd, sp);
+                     scale, idt, (const char*)samples, row_bytes, sf, sd, sp, msign, az, az_s, az_c);
   return SB_CHECK_LAUNCH();
 }
 

@@ -53,16 +53,20 @@ hipError_t launch_make_work(WorkItem* w, int nblocks, int cdp0, int ncdp, int nt
                             int npar, hipStream_t st);
 
 hipError_t launch_prep_traces(const TraceGeom* g, int ntr, float* h_ref32, double* h0, double* m0x, double* m0y,
-                              hipStream_t st);
+                              float* msign, hipStream_t st);
 // CMP slot tables of CDPs [cdp0, cdp0 + ncdp): each CDP's traces by increasing h.
 hipError_t launch_cmp_slots(int cdp0, int ncdp, const int32_t* cdp_off, const double* h0, const float* scale,
                             double idt, SlotF* slots, hipStream_t st);
 // CRS slot tables; the fast slots carry the byte address of their row
-// (samples + row * row_bytes).
+// (samples + row * row_bytes).  az != 0: m is the signed projection of the
+// midpoint displacement onto the direction (az_s, az_c) (sine and cosine of
+// the azimuth clockwise from +y) in true coordinates (msign: the sign of each
+// trace's reference coordinate factor), m^2 = m m; else the distance |dm|.
 hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
                             const int32_t* slot_off, const double* m0x, const double* m0y, const double* h0,
                             const float* scale, double idt, const void* samples, uint64_t row_bytes, SlotF* sf,
-                            SlotD* sd, SlotF* sp, hipStream_t st);
+                            SlotD* sd, SlotF* sp, const float* msign, int az, double az_s, double az_c,
+                            hipStream_t st);
 hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int store, void* dst, float* scale,
                         hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);

@@ -235,7 +235,7 @@ extern "C" void sb_gather_destroy(sb_gather* g) {
   // back to the cache for reuse, so wait for the whole device.
   (void)hipDeviceSynchronize();
   for (void* p : {(void*)g->d_samples_mem, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
-                  (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
+                  (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_msign, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
                   (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p})
     ctx_free(c, p);
   delete g;
@@ -319,6 +319,7 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
   G->ns = ns;
   G->dt_us = dt_us;
   G->store = store;
+  G->dm_azimuth_mdeg = ctx->dm_azimuth_mdeg;
   // >= 8 zero elements of padding after every row: 16-bit window loads may
   // read one dword past the last sample.
   G->stride = ((int64_t)ns + 8 + 63) / 64 * 64;
@@ -335,6 +336,7 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_h0))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0x))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * 8, (void**)&G->d_m0y))) return rc;
+  if ((rc = ctx_alloc(ctx, (size_t)ntr * 4, (void**)&G->d_msign))) return rc;
   if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(SlotF), (void**)&G->d_cmp_slots))) return rc;
   stage_rows = std::max<int64_t>(1, std::min<int64_t>(batch_rows, ntr));
   for (auto& st : stage)
@@ -352,7 +354,7 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
     if ((rc = ctx_alloc(ctx, (size_t)ntr * sizeof(TraceGeom), &dg))) return rc;
     hipError_t e = hipMemcpyAsync(dg, tg.data(), (size_t)ntr * sizeof(TraceGeom), hipMemcpyHostToDevice, cs);
     if (e == hipSuccess)
-      e = launch_prep_traces((const TraceGeom*)dg, ntr, G->d_h_ref32, G->d_h0, G->d_m0x, G->d_m0y, cs);
+      e = launch_prep_traces((const TraceGeom*)dg, ntr, G->d_h_ref32, G->d_h0, G->d_m0x, G->d_m0y, G->d_msign, cs);
     // tg is read by the copy: wait before it goes out of scope.
     if (e == hipSuccess) e = hipStreamSynchronize(cs);
     ctx_free(ctx, dg);
@@ -414,9 +416,12 @@ int GatherBuild::batch(int cb, int ce, hipEvent_t ev, int* ready) {
     // CDPs whose every neighbour lies below ce (the reach is a prefix maximum).
     int r = crs_ready;
     while (r < G->ncdp && nbr_reach[r] < ce) r++;
+    const bool az = G->dm_azimuth_mdeg >= 0;
+    const double th = az ? (double)G->dm_azimuth_mdeg / 1000.0 * (M_PI / 180.0) : 0.0;
     HIPR(launch_crs_slots(crs_ready, r - crs_ready, G->d_cdp_off, (const int32_t*)dnoff, (const int32_t*)dnbr,
                           G->d_crs_slot_off, G->d_m0x, G->d_m0y, G->d_h0, G->d_scale, idt, G->d_samples,
-                          (uint64_t)G->stride * esz, G->d_crs_slots, G->d_crs_slots_d, G->d_crs_slots_p, cs));
+                          (uint64_t)G->stride * esz, G->d_crs_slots, G->d_crs_slots_d, G->d_crs_slots_p, G->d_msign,
+                          az ? 1 : 0, std::sin(th), std::cos(th), cs));
     crs_ready = r;
     done = r;
   }
@@ -432,7 +437,7 @@ int GatherBuild::finish(sb_gather** out) {
   {
     std::lock_guard<std::mutex> lk(ctx->mem_mu);
     for (void* p : {(void*)g->d_samples_mem, (void*)g->d_scale, (void*)g->d_cdp_off, (void*)g->d_h_ref32, (void*)g->d_h0,
-                    (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
+                    (void*)g->d_m0x, (void*)g->d_m0y, (void*)g->d_msign, (void*)g->d_cmp_slots, (void*)g->d_crs_slot_off,
                     (void*)g->d_crs_slots, (void*)g->d_crs_slots_d, (void*)g->d_crs_slots_p}) {
       auto it = ctx->sizes.find(p);
       if (it != ctx->sizes.end()) g->bytes += it->second;
@@ -1108,6 +1113,11 @@ extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
     ctx->stream_rows = value;
     return SB_OK;
   }
+  if (option == SB_OPT_DM_AZIMUTH) {
+    if (value < -1 || value >= 360000) return fail(SB_ERR_INVALID, "SB_OPT_DM_AZIMUTH must be -1 or in [0, 360000)");
+    ctx->dm_azimuth_mdeg = value;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -1119,6 +1129,10 @@ extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
   }
   if (option == SB_OPT_STREAM_ROWS) {
     *value = ctx->stream_rows;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DM_AZIMUTH) {
+    *value = ctx->dm_azimuth_mdeg;
     return SB_OK;
   }
   return fail(SB_ERR_INVALID, "unknown option");

@@ -48,6 +48,7 @@ struct sb_ctx {
   void* queue_buf = nullptr;   // persistent-grid item counters
   int persistent = 0;          // SB_OPT_PERSISTENT
   int stream_rows = 0;         // SB_OPT_STREAM_ROWS (0: automatic)
+  int dm_azimuth_mdeg = -1;    // SB_OPT_DM_AZIMUTH (-1: the reference's unsigned distance)
   void* scratch = nullptr;     // device address of the static scratch on this device
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured
@@ -65,12 +66,14 @@ struct sb_gather {
   int32_t* d_cdp_off = nullptr;
   float* d_h_ref32 = nullptr;
   double *d_h0 = nullptr, *d_m0x = nullptr, *d_m0y = nullptr;
+  float* d_msign = nullptr;  // sign of each trace's reference coordinate factor
   sb::SlotF* d_cmp_slots = nullptr;
   int32_t* d_crs_slot_off = nullptr;
   sb::SlotF* d_crs_slots = nullptr;
   sb::SlotD* d_crs_slots_d = nullptr;
   sb::SlotF* d_crs_slots_p = nullptr;  // unscaled FP32 geometry (CRS-DE exact mode)
   int64_t ncrs_slots = 0;
+  int dm_azimuth_mdeg = -1;  // midpoint displacement of the slot tables (SB_OPT_DM_AZIMUTH at creation)
   std::vector<int32_t> cdp_off, crs_slot_off;  // host copies
   size_t bytes = 0;
 };

@@ -7,6 +7,7 @@
  *
  * Each routine cites the reference lines it restates.
  */
+#define _DEFAULT_SOURCE /* M_PI */
 #include "oracle.h"
 
 #include <math.h>
@@ -133,10 +134,12 @@ static void crs_geometry(int ntr, const orc_geom* g, double* m0x, double* m0y, d
 
 /* Supergather of one CDP: neighbour CDPs in list order, traces in gather
  * order, distances to the CDP's first trace (CRS/OpenMP/src/main.cpp:179-203,
- * 287-316).  Returns the size. */
-static int crs_supergather(int cdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
-                           const double* m0x, const double* m0y, const double* h0, int* rows, double* m2,
-                           double* m, double* h) {
+ * 287-316).  With dir = {sin, cos} of an azimuth (the library's
+ * SB_OPT_DM_AZIMUTH extension; the reference has none) m is the signed
+ * projection dx sin + dy cos and m2 = m m.  Returns the size. */
+static int crs_supergather_dir(int cdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
+                               const double* m0x, const double* m0y, const double* h0, const double* dir,
+                               const double* sgn, int* rows, double* m2, double* m, double* h) {
   const double rx = m0x[cdp_off[cdp]], ry = m0y[cdp_off[cdp]];
   int size = 0;
   for (int q = nbr_off[cdp]; q < nbr_off[cdp + 1]; q++) {
@@ -144,15 +147,34 @@ static int crs_supergather(int cdp, const int32_t* cdp_off, const int32_t* nbr_o
     for (int r = cdp_off[j]; r < cdp_off[j + 1]; r++) {
       double dx = m0x[r] - rx;
       double dy = m0y[r] - ry;
-      double mm2 = dx * dx + dy * dy;
       rows[size] = r;
-      m2[size] = mm2;
-      m[size] = sqrt(mm2);
+      if (dir) {
+        /* true coordinates: the reference's 1/scalco factor is negative for
+           a negative scalco (CRS/OpenMP/src/main.cpp:166-171) */
+        const double sr = sgn[cdp_off[cdp]], st = sgn[r];
+        dx = m0x[r] * st - rx * sr;
+        dy = m0y[r] * st - ry * sr;
+        double px = dx * dir[0];
+        double py = dy * dir[1];
+        double p = px + py;
+        m[size] = p;
+        m2[size] = p * p;
+      } else {
+        double mm2 = dx * dx + dy * dy;
+        m2[size] = mm2;
+        m[size] = sqrt(mm2);
+      }
       h[size] = h0[r];
       size++;
     }
   }
   return size;
+}
+
+static int crs_supergather(int cdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr,
+                           const double* m0x, const double* m0y, const double* h0, int* rows, double* m2,
+                           double* m, double* h) {
+  return crs_supergather_dir(cdp, cdp_off, nbr_off, nbr, m0x, m0y, h0, NULL, NULL, rows, m2, m, h);
 }
 
 static int max_supergather(int ncdp, const int32_t* cdp_off, const int32_t* nbr_off, const int32_t* nbr) {
@@ -218,7 +240,24 @@ int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc
                   const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1,
                   int nb, double c0, double c1, int nc, double taud, int cdp_begin, int cdp_end, int32_t* index,
                   float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem, float* panel_stk) {
+  return orc_crs_range_az(ncdp, cdp_off, ns, dt_us, g, samples, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0, c1, nc,
+                          taud, -1, cdp_begin, cdp_end, index, pa, pb, pc, coher, stack, panel_sem, panel_stk);
+}
+
+int orc_crs_range_az(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                     const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1,
+                     int nb, double c0, double c1, int nc, double taud, int az_mdeg, int cdp_begin, int cdp_end,
+                     int32_t* index, float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem,
+                     float* panel_stk) {
   if (cdp_begin < 0 || cdp_end > ncdp || cdp_begin >= cdp_end) return 1;
+  double dirv[2];
+  const double* dir = NULL;
+  if (az_mdeg >= 0) {
+    const double th = (double)az_mdeg / 1000.0 * (M_PI / 180.0);
+    dirv[0] = sin(th);
+    dirv[1] = cos(th);
+    dir = dirv;
+  }
   c0 = c0 * FACTOR;
   c1 = c1 * FACTOR;
   const double inc_a = (a1 - a0) * (1.0 / (double)na);
@@ -235,7 +274,9 @@ int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc
   double* m0x = (double*)malloc(sizeof(double) * ntr);
   double* m0y = (double*)malloc(sizeof(double) * ntr);
   double* h0 = (double*)malloc(sizeof(double) * ntr);
+  double* sgn = (double*)malloc(sizeof(double) * (ntr > 0 ? ntr : 1));
   crs_geometry(ntr, g, m0x, m0y, h0);
+  for (int i = 0; i < ntr; i++) sgn[i] = (g[i].scalco <= -1) ? -1.0 : 1.0; /* the factor 1/scalco is negative */
   const int maxs = max_supergather(ncdp, cdp_off, nbr_off, nbr);
   int* rows = (int*)malloc(sizeof(int) * maxs);
   double* m2 = (double*)malloc(sizeof(double) * maxs);
@@ -248,7 +289,7 @@ int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc
     pars[3 * i + 2] = c0 + inc_c * (i % nc);
   }
   for (int cdp = cdp_begin; cdp < cdp_end; cdp++) {
-    const int size = crs_supergather(cdp, cdp_off, nbr_off, nbr, m0x, m0y, h0, rows, m2, m, h);
+    const int size = crs_supergather_dir(cdp, cdp_off, nbr_off, nbr, m0x, m0y, h0, dir, sgn, rows, m2, m, h);
     const size_t row0 = (size_t)(cdp - cdp_begin) * ns;
     double stt00 = 0, sem00 = 0;
     crs_eval(pars[0], pars[1], pars[2], dt, idt, tau, w, ns, size, rows, m2, m, h, samples, 0, &sem00, &stt00);
@@ -290,6 +331,7 @@ int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc
   free(m);
   free(h);
   free(pars);
+  free(sgn);
   return 0;
 }
 

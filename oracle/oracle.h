@@ -40,6 +40,17 @@ int orc_crs_range(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc
                   int nb, double c0, double c1, int nc, double tau, int cdp_begin, int cdp_end, int32_t* index,
                   float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem, float* panel_stk);
 
+/* orc_crs_range with az_mdeg >= 0: the midpoint displacement m of every
+ * supergather trace is the signed projection of dm onto the azimuth az_mdeg
+ * (millidegrees clockwise from +y), m^2 = m m -- the library's
+ * SB_OPT_DM_AZIMUTH extension, which has no reference counterpart.  -1: as
+ * orc_crs_range. */
+int orc_crs_range_az(int ncdp, const int32_t* cdp_off, int ns, int dt_us, const orc_geom* g, const float* samples,
+                     const int32_t* nbr_off, const int32_t* nbr, double a0, double a1, int na, double b0, double b1,
+                     int nb, double c0, double c1, int nc, double tau, int az_mdeg, int cdp_begin, int cdp_end,
+                     int32_t* index, float* pa, float* pb, float* pc, float* coher, float* stack, float* panel_sem,
+                     float* panel_stk);
+
 /* FP64 oracle for error statements: the CMP search with every quantity in
  * double (geometry from the integer coordinates, grid, traveltime, sums).
  * Outputs as orc_cmp but in double. */

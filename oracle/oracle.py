@@ -31,6 +31,7 @@ def lib():
         L.orc_cmp.argtypes = [i, P, i, i, P, P, d, d, i, d, P, P, P, P, P, P]
         L.orc_crs.argtypes = [i, P, i, i, P, P, P, P, d, d, i, d, d, i, d, d, i, d, P, P, P, P, P, P, P, P]
         L.orc_crs_range.argtypes = L.orc_crs.argtypes[:18] + [i, i] + L.orc_crs.argtypes[18:]
+        L.orc_crs_range_az.argtypes = L.orc_crs.argtypes[:18] + [i, i, i] + L.orc_crs.argtypes[18:]
         L.orc_crs_de_range.argtypes = [i, P, i, i, P, P, P, P, d, d, d, d, d, d, d, i, i, d, d, C.c_uint64, i, i,
                                        P, P, P, P, P]
         L.orc_cmp_f64.argtypes = [i, P, i, i, P, P, d, d, i, d, P, P, P, P]
@@ -73,9 +74,11 @@ def cmp(cdp_off, geom, samples, dt_us, c0, c1, nc, tau, panel=False):
 
 
 def crs(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0, c1, nc, tau, panel=False,
-        cdp_range=None):
+        cdp_range=None, az_mdeg=-1):
     """orc_crs; cdp_range = (begin, end) searches only those CDPs (rows of the
-    outputs), their supergathers reaching any CDP of the gather."""
+    outputs), their supergathers reaching any CDP of the gather; az_mdeg >= 0
+    uses the signed projection of the midpoint displacement onto that
+    azimuth (orc_crs_range_az)."""
     cdp_off, samples, geom = _c(cdp_off, np.int32), _c(samples, np.float32), _geom(geom)
     nbr_off, nbr = _c(nbr_off, np.int32), _c(nbr if len(nbr) else [0], np.int32)
     ncdp, ns = len(cdp_off) - 1, samples.shape[1]
@@ -85,9 +88,9 @@ def crs(cdp_off, geom, samples, dt_us, nbr_off, nbr, a0, a1, na, b0, b1, nb, c0,
            for k in ("index", "a", "b", "c", "coher", "stack")}
     ps = np.zeros((ce - cb, ns, npar), np.float32) if panel else None
     pk = np.zeros((ce - cb, ns, npar), np.float32) if panel else None
-    rc = lib().orc_crs_range(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1, na,
-                             b0, b1, nb, c0, c1, nc, tau, cb, ce, _p(out["index"]), _p(out["a"]), _p(out["b"]),
-                             _p(out["c"]), _p(out["coher"]), _p(out["stack"]), _p(ps), _p(pk))
+    rc = lib().orc_crs_range_az(ncdp, _p(cdp_off), ns, dt_us, _p(geom), _p(samples), _p(nbr_off), _p(nbr), a0, a1,
+                                na, b0, b1, nb, c0, c1, nc, tau, az_mdeg, cb, ce, _p(out["index"]), _p(out["a"]),
+                                _p(out["b"]), _p(out["c"]), _p(out["coher"]), _p(out["stack"]), _p(ps), _p(pk))
     assert rc == 0
     if panel:
         out["panel_sem"], out["panel_stk"] = ps, pk

@@ -63,6 +63,18 @@ def crs_tau_samples(g, ref, t0, a, b, c, dt):
     return np.where(tt < 0, -1.0 / dt, np.sqrt(np.maximum(tt, 0)) / dt)
 
 
+def crs_tau_samples_projected(g, ref, t0, a, b, c, dt, azimuth_deg):
+    """The same with dm replaced by its signed projection onto the azimuth
+    (degrees clockwise from +y): m = dx sin + dy cos, m^2 = m m."""
+    mx, my = midpoint_m(g)
+    rx, ry = midpoint_m(ref)
+    th = np.deg2rad(azimuth_deg)
+    m = (mx - rx) * np.sin(th) + (my - ry) * np.cos(th)
+    h = half_offset_m(g)
+    tt = (t0 * dt + a * m) ** 2 + b * m * m + c * h * h
+    return np.where(tt < 0, -1.0 / dt, np.sqrt(np.maximum(tt, 0)) / dt)
+
+
 def ramp_stack(taus, alpha, beta):
     return beta + alpha * float(np.mean(taus))
 

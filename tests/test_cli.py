@@ -68,17 +68,38 @@ def test_cli_exact_outputs_are_byte_identical(manifest, tmp_path, case, prog):
 
 
 @pytest.mark.gpu
-def test_cli_crs_de_runs(manifest, tmp_path):
-    print("case: cli crs-de", flush=True)
+@pytest.mark.parametrize("azimuth,mode", [("0", None), ("90", "project")])
+def test_cli_crs_de_equals_library(sb, manifest, tmp_path, azimuth, mode):
+    """crs-de writes, byte for byte, the sections of sb_crs_de on the same
+    file, parameters and seed (population 32, F 0.5, CR 0.9, seed 1 by
+    default); with SB_AZIMUTH=project the supergathers use the signed
+    projection onto -azimuth (SB_OPT_DM_AZIMUTH)."""
+    print("case: cli crs-de, azimuth", azimuth, mode, flush=True)
     _copy_input("crs_w3", tmp_path)
     a = args_dict(manifest["crs_w3"]["args"])
-    argv = ["-ngen", "5", "-azimuth", "0", "-d", "1"]
+    argv = ["-ngen", "5", "-azimuth", azimuth, "-d", "1"]
     for k in ("-a0", "-a1", "-na", "-b0", "-b1", "-nb", "-c0", "-c1", "-nc", "-aph", "-apm", "-tau"):
         argv += [k, a[k]]
-    p = run(os.path.join(LIB, "crs-de"), argv + ["-i", "input.su", "-v", "2"], str(tmp_path))
+    env = {"SB_AZIMUTH": mode} if mode else {}
+    p = run(os.path.join(LIB, "crs-de"), argv + ["-i", "input.su", "-v", "2"], str(tmp_path), env)
     assert p.returncode == 0, p.stderr
-    for f in ("crs.a.su", "crs.b.su", "crs.c.su", "crs.coher.su", "crs.stack.su"):
-        assert os.path.getsize(str(tmp_path / f)) == os.path.getsize(os.path.join(GOLD, "crs_w3", f))
+    assert "Execution Time:" in p.stdout
+    G = sb.SuGather(str(tmp_path / "input.su"), sb.METHOD_CRS, int(a["-aph"]), int(a["-apm"]))
+    ctx = sb.Context(0)
+    try:
+        if mode:
+            ctx.set_option(sb.OPT_DM_AZIMUTH, int(float(azimuth) * 1000))
+        g = sb.Gather(ctx, G.cdp_off, G.geom, G.samples, G.dt_us, nbr_off=G.nbr_off, nbr=G.nbr)
+        prm = sb.de_params(float(a["-a0"]), float(a["-a1"]), float(a["-b0"]), float(a["-b1"]), float(a["-c0"]),
+                           float(a["-c1"]), float(a["-tau"]), pop=32, ngen=5, F=0.5, CR=0.9, seed=1)
+        out = sb.search(ctx, g, prm)
+        g.close()
+    finally:
+        ctx.close()
+    for key, f, rw in (("a", "crs.a.su", True), ("b", "crs.b.su", True), ("c", "crs.c.su", True),
+                       ("coher", "crs.coher.su", False), ("stack", "crs.stack.su", False)):
+        G.write_section(str(tmp_path / ("lib_" + f)), out[key], rw)
+        assert filecmp.cmp(str(tmp_path / f), str(tmp_path / ("lib_" + f)), shallow=False), f
 
 
 @pytest.mark.gpu

@@ -223,3 +223,13 @@ def test_serving_workload_alone_equals_mix(sb, gpu_ctx, store):
         assert out is not None, rs
         for k in ("index", "a", "b", "c", "coher", "stack"):
             assert np.array_equal(out[k], r[k]), (i, reqs[i][0], k)
+
+
+def test_device_fp32_fma_rate(sb):
+    """The FP32 FMA rate the device sustains without packed FP32: the ceiling
+    the search kernels' arithmetic is held to (printed; the published 157.3
+    TFLOP/s counts packed FP32)."""
+    print("case: fp32 fma rate probe", flush=True)
+    t = sb.fp32_fma_rate(0)
+    print("sustained FP32 FMA rate: %.1f TFLOP/s" % t, flush=True)
+    assert 10.0 < t < 200.0

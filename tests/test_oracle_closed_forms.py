@@ -122,6 +122,40 @@ def test_crs_ramp_stack_matches_moveout(orc):
             assert sem64 == pytest.approx(out["coher"][cdp, t0], rel=1e-6)
 
 
+@pytest.mark.parametrize("az", [90.0, 30.0, 0.0])
+def test_crs_projected_midpoint_ramp_stack(orc, az):
+    """The azimuth extension (orc_crs_range_az): with m the signed projection
+    of the midpoint displacement onto the azimuth, the ramp stack is the mean
+    moveout of the projected traveltimes.  Midpoints on both sides of the
+    reference trace: a sign or sine/cosine mistake moves the taus.  At
+    azimuth 0 (north) the x displacements project to zero."""
+    mids = [(0.0, 0.0), (-12.0, 1.0), (9.0, -2.5), (25.0, 3.0), (-30.0, -1.0)]
+    hs = [40.0, 150.0, 300.0, 80.0, 420.0]
+    g, s = cf.line_gather(hs, NS, "ramp", alpha=0.02, beta=-0.5, mids=mids)
+    off = np.array([0, 5], np.int32)
+    nbr_off, nbr = np.array([0, 1], np.int32), np.array([0], np.int32)
+    a0, a1, b0, b1, c0, c1 = -0.7e-3, 0.7e-3, -1e-7, 1e-7, 1.975e-7, 1.384e-6
+    out = orc.crs(off, g, s, DT, nbr_off, nbr, a0, a1, 4, b0, b1, 2, c0, c1, 3, TAU, az_mdeg=int(az * 1000))
+    dt = DT * 1e-6
+    checked = 0
+    for t0 in range(0, NS, 5):
+        idx = out["index"][0, t0]
+        if out["coher"][0, t0] <= 0:
+            continue
+        ia, ib, ic = idx // 6, (idx // 3) % 2, idx % 3
+        A = a0 + (a1 - a0) / 4 * ia
+        B = b0 + (b1 - b0) / 2 * ib
+        Cc = c0 + (c1 - c0) / 3 * ic
+        taus = cf.crs_tau_samples_projected(g, g[0], t0, A, B, Cc, dt, az)
+        taus = taus[cf.valid_mask(taus, NS, 1)]
+        assert out["stack"][0, t0] == pytest.approx(cf.ramp_stack(taus, 0.02, -0.5), rel=1e-6, abs=1e-6), (t0, idx)
+        checked += 1
+    assert checked > 25  # of the 41 t0 visited
+    if az != 0.0:  # the signed projection differs from the distance somewhere
+        ref = orc.crs(off, g, s, DT, nbr_off, nbr, a0, a1, 4, b0, b1, 2, c0, c1, 3, TAU)
+        assert not np.array_equal(ref["stack"], out["stack"])
+
+
 def test_crs_no_winner_reports_first_parameter(orc):
     g, s = cf.line_gather([0.0, 0.0], NS, "flip")
     off = np.array([0, 2], np.int32)
