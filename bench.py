@@ -388,7 +388,7 @@ def serving_lines(sb, seed=SEED + 4):
     return lines
 
 
-def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, store=0):
+def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEED + 4, store=0, deadline_ms=0.0):
     """Serving loop under load on the resident lines of serving_lines():
     requests name a line and a CDP range.  Poisson arrivals at `rate` of CMP
     (70 %) and CRS (30 %) panel requests plus a probe stream of 1-CDP CMP
@@ -427,6 +427,7 @@ def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEE
         # warm-up: one request of each kind
         for ev in events[:4]:
             srv.wait(srv.submit_range(gid[ev[2]], ev[3], ev[4], ev[5]))
+        srv.set_deadline(deadline_ms)
         print("bench: serving %d requests over %.1f s (%.0f + %.0f req/s offered)" %
               (len(events), seconds, rate, probe_rate), flush=True)
         start = time.perf_counter()
@@ -438,20 +439,23 @@ def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEE
         lat = {"cmp": [], "crs": [], "probe": []}
         wait = {"cmp": [], "crs": [], "probe": []}
         failed = 0
-        last_done = 0.0
+        shed = {"cmp": 0, "crs": 0, "probe": 0}
         for kind, tk in tickets:
             out, st = srv.wait(tk)
             if out is not None:
                 lat[kind].append(st["t_done_ms"] - st["t_submit_ms"])
                 wait[kind].append(st["t_admit_ms"] - st["t_submit_ms"])
+            elif st["status"] == sb.ERR_DEADLINE:
+                shed[kind] += 1
             else:
                 failed += 1
         wall = time.perf_counter() - start
         stats = srv.stats()
+    nshed = sum(shed.values())
     res = dict(store="f32" if store == 0 else "f16", offered_rps=rate + probe_rate, duration_s=round(wall, 3),
-               requests=len(events), served=stats["done"] - 4, failed=failed,
-               served_rps=round((stats["done"] - 4) / wall, 1), launches=stats["launches"] - 4,
-               resident_gb=round(resident / 1e9, 2))
+               requests=len(events), served=stats["done"] - 4 - nshed - failed, shed=nshed, failed=failed,
+               served_rps=round((stats["done"] - 4 - nshed - failed) / wall, 1), launches=stats["launches"] - 4,
+               resident_gb=round(resident / 1e9, 2), deadline_ms=deadline_ms)
     for k, v in lat.items():
         v = np.array(v)
         if len(v):
@@ -462,23 +466,27 @@ def serving_bench(sb, lines, seconds=4.0, rate=200.0, probe_rate=100.0, seed=SEE
     return res
 
 
-def serving_sweep(sb, lines, rates=(200, 500, 1000, 2000), seconds=2.0):
-    """The serving workload over its arrival-rate range, both stores: per
-    offered rate, served vs offered, p50/p99 per request kind, and the
-    lowest offered rate at which the 1-CDP probe's p99 exceeds 50 ms."""
+def serving_sweep(sb, lines, rates=(200, 500, 1000, 2000), seconds=2.0, deadline_ms=25.0):
+    """The serving workload over its arrival-rate range: per offered rate,
+    served (and shed) vs offered, p50/p99 per request kind of the served
+    requests, and the lowest offered rate at which the 1-CDP probe's p99
+    exceeds 50 ms -- for both stores with every request queued, and for FP32
+    with an admission deadline (load shedding)."""
     out = {}
-    for store, name in ((0, "f32"), (1, "f16")):
+    for name, store, dl in (("f32", 0, 0.0), ("f16", 1, 0.0), ("f32_deadline", 0, deadline_ms)):
         curve = []
         for r in rates:
-            res = serving_bench(sb, lines, seconds=seconds, rate=float(r), store=store)
+            res = serving_bench(sb, lines, seconds=seconds, rate=float(r), store=store, deadline_ms=dl)
             curve.append(dict(offered_rps=res["offered_rps"], requests=res["requests"], served=res["served"],
-                              failed=res["failed"], served_rps=res["served_rps"], duration_s=res["duration_s"],
+                              shed=res["shed"], failed=res["failed"], served_rps=res["served_rps"],
+                              duration_s=res["duration_s"],
                               **{k: {q: round(res[k][q], 2) for q in ("p50_ms", "p99_ms", "queue_p99_ms")}
                                  for k in ("probe", "cmp", "crs") if k in res}))
         crossing = next((c["offered_rps"] for c in curve if c.get("probe", {}).get("p99_ms", 0) > 50.0), None)
-        out[name] = dict(curve=curve, probe_p99_over_50ms_from_rps=crossing)
+        out[name] = dict(curve=curve, deadline_ms=dl, probe_p99_over_50ms_from_rps=crossing)
     out["what"] = ("offered = mixed CMP/CRS panel requests at the named rate + 100 req/s of 1-CDP CMP probes, %.0f s "
-                   "of arrivals per point; latency = done - submit" % seconds)
+                   "of arrivals per point; latency = done - submit of the served requests; f32_deadline: requests "
+                   "not admitted within %.0f ms are shed (SB_ERR_DEADLINE)" % (seconds, deadline_ms))
     return out
 
 

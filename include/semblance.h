@@ -36,7 +36,8 @@ enum {
   SB_ERR_BUDGET = 5,      /* request exceeds the ctx HBM budget            */
   SB_ERR_NOMEM = 6,       /* host allocation failed                        */
   SB_ERR_UNSUPPORTED = 7, /* e.g. window w = 2*tau+1 above SB_MAX_W        */
-  SB_ERR_STATE = 8        /* object used after shutdown / wrong method     */
+  SB_ERR_STATE = 8,       /* object used after shutdown / wrong method     */
+  SB_ERR_DEADLINE = 9     /* serving: not admitted within the deadline     */
 };
 
 #define SB_MAX_W 15       /* largest coherence window (2*tau+1) supported   */
@@ -435,6 +436,13 @@ int sb_server_add_gather(sb_server* s, int ncdp, const int32_t* cdp_off, int ns,
 /* Rejects new requests naming the gather, waits for those queued or running,
  * then frees it. */
 int sb_server_remove_gather(sb_server* s, int gather_id);
+/* Load shedding: a request that has waited max_queue_ms since its
+ * submission without being admitted fails with SB_ERR_DEADLINE (its
+ * sb_server_wait returns that status, within about max_queue_ms / 8 of the
+ * deadline) instead of queueing; admitted requests always complete.
+ * 0 (default): no deadline, every request waits its turn.  Requests already
+ * queued are held to the new value. */
+int sb_server_set_deadline(sb_server* s, double max_queue_ms);
 /* hbm_used: bytes held by running requests and resident gathers. */
 int sb_server_stats(const sb_server* s, uint64_t* done, uint64_t* launches, size_t* hbm_used);
 /* Stops the workers; queued requests that have not started fail. */

@@ -19,6 +19,7 @@ OPT_PERSISTENT = 1
 OPT_STREAM_ROWS = 2
 OPT_DM_AZIMUTH = 3
 GRAPH_CRS_DE = 2
+ERR_DEADLINE = 9
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
 
 GEOM_DTYPE = np.dtype([("cdp", "<i4"), ("scalco", "<i4"), ("sx", "<i4"), ("sy", "<i4"), ("gx", "<i4"),
@@ -101,7 +102,7 @@ EXPORTS = [
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
     "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
-    "sb_search_streamed", "sb_cmp_run_noalloc", "sb_device_fp32_fma_rate",
+    "sb_search_streamed", "sb_cmp_run_noalloc", "sb_device_fp32_fma_rate", "sb_server_set_deadline",
 ]
 
 
@@ -182,6 +183,7 @@ def lib():
         L.sb_graph_destroy.restype = None
         L.sb_cmp_run_noalloc.argtypes = [P, P, C.POINTER(CmpParams), C.c_int, C.c_int, C.POINTER(DeviceArrays)]
         L.sb_device_fp32_fma_rate.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        L.sb_server_set_deadline.argtypes = [P, C.c_double]
         _lib = L
     return _lib
 
@@ -559,6 +561,10 @@ class Server:
         if rc != 0:
             return None, dict(stats, error=lib().sb_strerror(rc).decode())
         return out, stats
+
+    def set_deadline(self, max_queue_ms):
+        """Requests not admitted within max_queue_ms fail (0: no deadline)."""
+        check(lib().sb_server_set_deadline(self.h, float(max_queue_ms)))
 
     def stats(self):
         d, l, u = C.c_uint64(), C.c_uint64(), C.c_size_t()

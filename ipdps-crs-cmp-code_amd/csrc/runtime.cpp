@@ -168,6 +168,7 @@ extern "C" const char* sb_strerror(int code) {
     case SB_ERR_NOMEM: return "out of host memory";
     case SB_ERR_UNSUPPORTED: return "unsupported configuration";
     case SB_ERR_STATE: return "invalid state";
+    case SB_ERR_DEADLINE: return "admission deadline passed";
     default: return "unknown error";
   }
 }

@@ -10,7 +10,11 @@
 //   - a slot merges the waiting requests that share its request's shape
 //     (method, ns, dt, parameters, store) into one gather and one launch, up
 //     to max_batch CDPs, and splits the results back per request;
-//   - sb_server_wait blocks for one ticket and copies its results out.
+//   - sb_server_wait blocks for one ticket and copies its results out;
+//   - with an admission deadline (sb_server_set_deadline) a reaper thread
+//     fails requests that waited longer than it without being admitted
+//     (load shedding: under overload the admitted requests keep a bounded
+//     latency instead of every request queueing behind the backlog).
 // Lines registered with sb_server_add_gather stay on the device; requests
 // that name one and a CDP range skip the upload, and the requests merged
 // into one batch run as range searches on the slot's stream into one result
@@ -135,6 +139,8 @@ struct sb_server {
   std::map<uint64_t, std::shared_ptr<Req>> all;
   uint64_t next_ticket = 1, done = 0, launches = 0;
   bool stop = false;
+  double deadline_ms = 0;  // 0: no admission deadline
+  std::condition_variable cv_reap;
   std::vector<std::thread> workers;
   std::vector<sb_ctx*> ctx;
   sb_ctx* res_ctx = nullptr;  // owns the resident gathers
@@ -163,6 +169,7 @@ struct sb_server {
   int next_gid = 1;
 
   void worker(int slot);
+  void reaper();
   int process(int slot, std::vector<std::shared_ptr<Req>>& batch);
   int process_range(int slot, std::vector<std::shared_ptr<Req>>& batch);
 };
@@ -288,6 +295,35 @@ int sb_server::process(int slot, std::vector<std::shared_ptr<Req>>& batch) {
   return rc;
 }
 
+// Fails the pending requests that waited past the admission deadline
+// (under the server lock).
+void sb_server::reaper() {
+  std::unique_lock<std::mutex> lk(mu);
+  while (!stop) {
+    if (deadline_ms <= 0) {
+      cv_reap.wait(lk);
+      continue;
+    }
+    const double now = now_ms(t0);
+    bool any = false;
+    for (auto it = pending.begin(); it != pending.end();) {
+      auto& q = *it;
+      if (now - q->t_submit > deadline_ms) {
+        q->status = SB_ERR_DEADLINE;
+        q->t_admit = q->t_done = now;
+        if (q->res) q->res->users--;
+        done++;
+        any = true;
+        it = pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    if (any) cv_done.notify_all();
+    cv_reap.wait_for(lk, std::chrono::duration<double, std::milli>(std::max(0.25, deadline_ms / 8)));
+  }
+}
+
 void sb_server::worker(int slot) {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
@@ -397,6 +433,7 @@ extern "C" int sb_server_create(int device, size_t hbm_budget, int max_batch, sb
   sb_ctx_memory(s->ctx[0], nullptr, &b);
   s->budget = b;
   for (int i = 0; i < kSlots; i++) s->workers.emplace_back([p = s.get(), i] { p->worker(i); });
+  s->workers.emplace_back([p = s.get()] { p->reaper(); });
   *out = s.release();
   return SB_OK;
 }
@@ -550,6 +587,14 @@ extern "C" int sb_server_remove_gather(sb_server* s, int gather_id) {
   return SB_OK;
 }
 
+extern "C" int sb_server_set_deadline(sb_server* s, double max_queue_ms) {
+  if (!s || !(max_queue_ms >= 0)) return fail(SB_ERR_INVALID, "sb_server_set_deadline: bad arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->deadline_ms = max_queue_ms;
+  s->cv_reap.notify_all();
+  return SB_OK;
+}
+
 extern "C" int sb_server_stats(const sb_server* cs, uint64_t* done, uint64_t* launches, size_t* hbm_used) {
   sb_server* s = const_cast<sb_server*>(cs);
   if (!s) return fail(SB_ERR_INVALID, "null server");
@@ -575,6 +620,7 @@ extern "C" void sb_server_destroy(sb_server* s) {
   }
   s->cv_work.notify_all();
   s->cv_done.notify_all();
+  s->cv_reap.notify_all();
   for (auto& t : s->workers) t.join();
   for (auto& kv : s->resident) sb_gather_destroy(kv.second.g);
   s->resident.clear();

@@ -33,8 +33,9 @@ def test_library_has_gfx950_code_object():
 
 def test_error_strings(sb):
     L = sb.lib()
-    for code in range(0, 9):
-        assert L.sb_strerror(code)
+    for code in range(0, 10):
+        assert L.sb_strerror(code) and L.sb_strerror(code) != b"unknown error", code
+    assert L.sb_strerror(9) == b"admission deadline passed"
 
 
 @pytest.mark.skipif(has_gpu(), reason="checks the no-device error path")

@@ -185,3 +185,35 @@ def test_resident_gathers_alone_and_in_the_mix(sb, gpu_ctx):
             want = {k: v[cb:ce] for k, v in ref[kind].items()}
         for k in ("index", "a", "b", "c", "coher", "stack"):
             assert np.array_equal(out[k], want[k]), (kind, cb, ce, k)
+
+
+def test_admission_deadline_sheds_load(sb, gpu_ctx):
+    """With an admission deadline, a burst far above the GPU's capacity is
+    shed: requests not admitted within the deadline fail with
+    SB_ERR_DEADLINE, the admitted ones complete with their alone results, and
+    none waited much longer than the deadline before admission.  Without a
+    deadline every request of the same burst is served."""
+    print("case: serving admission deadline", flush=True)
+    line = sb.synth(0, 64, (200, 400), 2001, 2000, 77, h_max=1500.0, h_jitter=5.0, nevents=6)
+    prm = sb.cmp_params(1.98e-7, 1.77e-6, 128, 0.002)
+    g = sb.Gather(gpu_ctx, *line, 2000)
+    alone = sb.search(gpu_ctx, g, prm, (0, 64))
+    g.close()
+    for deadline in (20.0, 0.0):
+        with sb.Server(0, max_batch=1) as srv:
+            gid = srv.add_gather(*line, 2000)
+            srv.set_deadline(deadline)
+            tickets = [srv.submit_range(gid, 0, 64, prm) for _ in range(80)]  # ~80 x 5 ms of GPU work at once
+            res = [srv.wait(t) for t in tickets]
+        served = [(o, st) for o, st in res if o is not None]
+        shed = [st for o, st in res if o is None]
+        print("deadline %.0f ms: %d served, %d shed" % (deadline, len(served), len(shed)), flush=True)
+        for o, st in served:
+            for k in ("index", "coher", "stack", "c"):
+                assert np.array_equal(o[k], alone[k]), k
+        if deadline > 0:
+            assert len(shed) > 0 and len(served) > 0
+            assert all(st["status"] == sb.ERR_DEADLINE for st in shed)
+            assert all(st["t_admit_ms"] - st["t_submit_ms"] <= deadline + 10.0 for o, st in served)
+        else:
+            assert not shed
