@@ -264,7 +264,10 @@ int sb_crs_de(sb_ctx* ctx, const sb_gather* g, const sb_de_params* p, sb_host_re
 
 /* Timing of the last *_run on this ctx, from HIP events recorded on the ctx
  * stream around each kernel.  tuples = sum over CDPs of traces x ns x params
- * (the reference's "Semblances-Trace" count). */
+ * (the reference's "Semblances-Trace" count).  After sb_search_streamed:
+ * search_ms = the summed device time of its searches (the counterpart of the
+ * CUDA programs' summed kernel events, CMP/CUDA/src/main.cu:306-326),
+ * finalize_ms = 0, total_ms = the whole call. */
 typedef struct {
   double search_ms;    /* main search kernel                     */
   double finalize_ms;  /* winner re-evaluation / output kernel   */

@@ -1,6 +1,7 @@
 // cmp: drop-in for the reference CMP programs (cmp-omp2 / cmp-cuda,
 // CMP/OpenMP/src/main.cpp:51-278): same flags, same outputs
-// (cmp.c.su, cmp.coher.su, cmp.stack.su in the working directory), same INFO line.
+// (cmp.c.su, cmp.coher.su, cmp.stack.su in the working directory), same INFO
+// line (the CUDA program's form when run as cmp-cuda or with SB_INFO=cuda).
 #include "cli_common.hpp"
 
 int main(int argc, const char** argv) {
@@ -33,8 +34,7 @@ int main(int argc, const char** argv) {
   HostOut out((size_t)s.ncdp * s.ns);
   check(s.search(SB_METHOD_CMP, &prm, &out.view), "CMP search");
   const double t = now_s() - t0;
-  const double stps = ((double)s.ntr / 1e9) * s.ns * nc / t;
-  log(INFO, "Execution Time: " + std::to_string(t) + ", Semblances-Trace/s: " + std::to_string(stps));
+  log_rate(argv[0], s, t, (double)s.ntr * s.ns * nc, "Semblances-Trace/s");
   write(s, "cmp.c.su", out.c, 1);
   write(s, "cmp.coher.su", out.coher, 0);
   write(s, "cmp.stack.su", out.stack, 0);

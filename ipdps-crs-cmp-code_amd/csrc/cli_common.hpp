@@ -178,6 +178,36 @@ struct HostOut {
   }
 };
 
+// The throughput INFO line.  By default the OpenMP programs' form,
+// "Execution Time: T, <label>: X" (CMP/OpenMP/src/main.cpp:239-241, label
+// "Semblances-Trace/s"; CRS/OpenMP/src/main.cpp:326-328, "Mega-Semblances-
+// Trace/s" with a giga value).  Run under a name ending in "-cuda" (e.g. a
+// cmp-cuda link to cmp) or with SB_INFO=cuda, the CUDA programs' form
+// (CMP/CUDA/src/main.cu:316-325, CRS/CUDA/src/main.cu:471-478), whose kernel
+// time is the summed device time of the searches.  evals = the reference's
+// Semblances-Trace count.
+inline bool cuda_info(const char* argv0) {
+  const char* e = std::getenv("SB_INFO");
+  if (e && !std::strcmp(e, "cuda")) return true;
+  const size_t n = std::strlen(argv0);
+  return n >= 5 && !std::strcmp(argv0 + n - 5, "-cuda");
+}
+
+inline void log_rate(const char* argv0, const Session& s, double t, double evals, const char* omp_label) {
+  if (!cuda_info(argv0)) {
+    log(INFO, "Execution Time: " + std::to_string(t) + ", " + omp_label + ": " + std::to_string(evals / 1e9 / t));
+    return;
+  }
+  sb_timing tm;
+  check(sb_last_timing(s.ctx, &tm), "timing");
+  const double k = tm.search_ms / 1e3;
+  std::string m = "Total Execution Time: " + std::to_string(t);
+  m += ": Giga-Semblances-Trace/s: " + std::to_string(evals / 1e9 / t);
+  m += ": Kernel Execution Time: " + std::to_string(k);
+  m += ": Kernel Giga-Semblances-Trace/s: " + std::to_string(k > 0 ? evals / 1e9 / k : 0.0);
+  log(INFO, m);
+}
+
 inline void write(const Session& s, const char* path, const std::vector<float>& v, int rewrite) {
   check(sb_su_write_section(s.su, path, v.data(), rewrite), path);
 }

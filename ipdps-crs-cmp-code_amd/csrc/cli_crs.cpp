@@ -1,6 +1,7 @@
 // crs: drop-in for the reference CRS programs (crs-omp2 / crs-cuda,
 // CRS/OpenMP/src/main.cpp:59-379): same flags, same outputs
-// (crs.{a,b,c,coher,stack}.su in the working directory), same INFO line.
+// (crs.{a,b,c,coher,stack}.su in the working directory), same INFO line (the
+// CUDA program's form when run as crs-cuda or with SB_INFO=cuda).
 #include "cli_common.hpp"
 
 int main(int argc, const char** argv) {
@@ -49,8 +50,7 @@ int main(int argc, const char** argv) {
   check(s.search(SB_METHOD_CRS, &prm, &out.view), "CRS search");
   const double t = now_s() - t0;
   const double npar = (double)prm.na * prm.nb * prm.nc;
-  const double stps = (s.traces_per_eval(true) / 1e9) * (s.ns * npar / t);
-  log(INFO, "Execution Time: " + std::to_string(t) + ", Mega-Semblances-Trace/s: " + std::to_string(stps));
+  log_rate(argv[0], s, t, s.traces_per_eval(true) * s.ns * npar, "Mega-Semblances-Trace/s");
   write(s, "crs.a.su", out.a, 1);
   write(s, "crs.b.su", out.b, 1);
   write(s, "crs.c.su", out.c, 1);

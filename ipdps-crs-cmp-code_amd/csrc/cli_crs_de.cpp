@@ -92,8 +92,7 @@ int main(int argc, const char** argv) {
   check(s.search(SB_GRAPH_CRS_DE, &prm, &out.view), "CRS-DE search");
   const double t = now_s() - t0;
   const double evals = (double)prm.pop * (prm.ngen + 1);
-  const double stps = (s.traces_per_eval(true) / 1e9) * (s.ns * evals / t);
-  log(INFO, "Execution Time: " + std::to_string(t) + ", Mega-Semblances-Trace/s: " + std::to_string(stps));
+  log_rate(argv[0], s, t, s.traces_per_eval(true) * s.ns * evals, "Mega-Semblances-Trace/s");
   write(s, "crs.a.su", out.a, 1);
   write(s, "crs.b.su", out.b, 1);
   write(s, "crs.c.su", out.c, 1);

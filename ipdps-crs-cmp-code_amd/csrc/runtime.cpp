@@ -1225,9 +1225,12 @@ extern "C" int sb_search_streamed(sb_ctx* ctx, int kind, const void* params, int
   if (kind != SB_METHOD_CMP && (!nbr_off || !nbr))
     return fail(SB_ERR_INVALID, "sb_search_streamed: CRS needs neighbour lists");
   HIPR(hipSetDevice(ctx->device));
+  const auto t_call = std::chrono::steady_clock::now();
   hipStream_t cs = nullptr;
   HIPR(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   std::vector<hipEvent_t> evs;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // device span of each range search
+  double tuples = 0;
   sb_result* r = nullptr;
   int rc = SB_OK;
   {
@@ -1268,12 +1271,22 @@ extern "C" int sb_search_streamed(sb_ctx* ctx, int kind, const void* params, int
         view.stack += shift;
         view.keys += shift;
         const sb_gather* g = b.G.get();
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+          if (e0) (void)hipEventDestroy(e0);
+          rc = fail(SB_ERR_HIP, "event");
+          break;
+        }
+        spans.push_back({e0, e1});
+        (void)hipEventRecord(e0, ctx->stream);  // fires once the batch's copy has landed
         if (kind == SB_METHOD_CMP)
           rc = sb_cmp_run_range(ctx, g, (const sb_cmp_params*)params, searched, ready, &view);
         else if (kind == SB_METHOD_CRS)
           rc = sb_crs_run_range(ctx, g, (const sb_crs_params*)params, searched, ready, &view);
         else
           rc = sb_crs_de_run_range(ctx, g, (const sb_de_params*)params, searched, ready, &view);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (!rc) tuples += ctx->last.tuples;
         searched = ready;
       }
       cb = ce;
@@ -1289,6 +1302,24 @@ extern "C" int sb_search_streamed(sb_ctx* ctx, int kind, const void* params, int
   }
   if (r) sb_result_destroy(r);
   (void)hipStreamSynchronize(ctx->stream);
+  // sb_last_timing after a streamed search: the summed device time of its
+  // range searches (key clear, search, winner re-evaluation; the uploads
+  // overlap them) and the whole call.
+  double dev_ms = 0;
+  for (auto& sp : spans) {
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, sp.first, sp.second) == hipSuccess) dev_ms += ms;
+    (void)hipEventDestroy(sp.first);
+    (void)hipEventDestroy(sp.second);
+  }
+  if (!rc) {
+    ctx->last = sb_timing{};
+    ctx->last.search_ms = dev_ms;
+    ctx->last.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
+    ctx->last.tuples = tuples;
+    ctx->last.variant = -1;
+    ctx->timing_pending = false;
+  }
   for (hipEvent_t ev : evs) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(cs);
   return rc;

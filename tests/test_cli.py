@@ -115,3 +115,21 @@ def test_cli_reads_segy_input(manifest, tmp_path, case, prog):
     assert p.returncode == 0, p.stderr
     for f in c["outputs"]:
         assert filecmp.cmp(str(tmp_path / f), os.path.join(GOLD, case, f), shallow=False), (case, f)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,prog", [("cmp_w3", "cmp"), ("crs_w3", "crs")])
+def test_cli_cuda_info_line(manifest, tmp_path, case, prog):
+    """SB_INFO=cuda: the CUDA programs' INFO line (CMP/CUDA/src/main.cu:316-325),
+    kernel time = the summed device time of the searches, within the total."""
+    import re
+    print("case: cli cuda info", case, flush=True)
+    _copy_input(case, tmp_path)
+    c = manifest[case]
+    p = run(os.path.join(LIB, prog), c["args"] + ["-i", "input.su", "-v", "2"], str(tmp_path), {"SB_INFO": "cuda"})
+    assert p.returncode == 0, p.stderr
+    m = re.search(r"Total Execution Time: ([0-9.]+): Giga-Semblances-Trace/s: ([0-9.]+): Kernel Execution Time: "
+                  r"([0-9.]+): Kernel Giga-Semblances-Trace/s: ([0-9.]+)", p.stdout)
+    assert m, p.stdout
+    total, rate, kern, krate = (float(x) for x in m.groups())
+    assert 0 < kern <= total and krate >= rate > 0
