@@ -96,6 +96,29 @@ def test_lds_partial_staging(sb, gpu_ctx, store):
         g.close()
 
 
+def test_lds_far_offsets_stage_nothing(sb, gpu_ctx):
+    """Half-offsets of 3-4 km, 20001 samples at 0.5 ms and C from 1e-8 to
+    5e-6: many workgroups cannot stage even their nearest trace's range (K =
+    0, every window from global memory; tests/lds_replay.py counts them on the
+    CPU), the others stage some; results equal the global-memory kernel's."""
+    import lds_replay as lr
+    print("case: lds far offsets (K = 0)", flush=True)
+    off, geom, s = lr.far_offset_line()
+    g = sb.Gather(gpu_ctx, off, geom, s, 500)
+    try:
+        prm = sb.cmp_params(1e-8, 5e-6, 16, 0.001)
+        ref, _ = _search(sb, gpu_ctx, g, prm, 1)
+        assert (ref["index"] >= 0).mean() > 0.05
+        for v in _lds_variants(sb, gpu_ctx):
+            out, t = _search(sb, gpu_ctx, g, prm, v, 16)
+            assert t["lds_staged"] == 1, (v, t)
+            for k in KEYS:
+                assert np.array_equal(out[k], ref[k]), (v, k)
+    finally:
+        gpu_ctx.tune_set(0, 0)
+        g.close()
+
+
 def test_lds_falls_back_when_it_cannot_stage(sb, gpu_ctx):
     """A CDP with more traces than the slab table holds runs on the global
     kernel (same results), and CRS never uses the LDS variants."""

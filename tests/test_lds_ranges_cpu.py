@@ -87,3 +87,27 @@ def test_lds_staged_ranges_cover_every_window():
     ]
     for case in cases:
         assert _check(rng, *case) == 0, case
+
+
+def test_staging_cases_reached_by_the_gpu_tests(sb):
+    """The staging decision replayed on the CPU for the gathers and launch
+    shapes of tests/test_gpu_lds.py: between them the GPU tests run
+    workgroups that stage every trace, workgroups that stage only their first
+    traces and workgroups that stage none (K = 0, every window from global
+    memory; the far-offset case)."""
+    import lds_replay as lr
+    seen = dict(full=0, partial=0, none=0, dead=0)
+    # test_lds_partial_staging: headline geometry, 128 and 256 parameters per workgroup, 40-64 KiB
+    off, geom, _ = sb.synth(0, 5, (60, 120), 1501, 2000, 31, h_max=1500.0, h_jitter=5.0, nevents=8)
+    for kb in (40, 48, 52, 64):
+        for pc in (128, 256):
+            c = lr.staging_cases(off, geom, 2000, 1501, 0.002, 1.98e-7, 1.77e-6, 256, pc, kb)
+            for k in seen:
+                seen[k] += c[k]
+    part = dict(seen)
+    # test_lds_far_offsets_stage_nothing
+    off, geom, _ = lr.far_offset_line()
+    c = lr.staging_cases(off, geom, 500, 20001, 0.001, 1e-8, 5e-6, 16, 16, 40)
+    print("partial-staging gathers: %s; far-offset gather: %s" % (part, c))
+    assert part["full"] > 0 and part["partial"] > 0
+    assert c["none"] > 0
