@@ -594,6 +594,7 @@ def main():
     f16 = results.get("f16")
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
     fma_rate = sb.fp32_fma_rate(0)  # the device's sustained FP32 FMA rate without packed FP32
+    win_rates = sb.window_rates(0)  # the inner step's instruction forms, window samples/s
     skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
     lds = bool(f32["lds_staged"])
     kernel = "k_search_lds" if lds else "k_search_fast"
@@ -637,13 +638,18 @@ def main():
                      "frac_reference_count": achieved / PEAK_FP32_VECTOR_TFLOPS,
                      "fp32_fma_rate_measured": fma_rate,
                      "frac_of_measured_fma_rate": achieved * (1.0 - skip) / fma_rate,
+                     "window_sample_rates": win_rates,
                      "note": "gather-reduce on the FP32 vector ALUs (no MFMA); peak = MI355X FP32 157.3 TFLOP/s, "
                              "which counts packed FP32 -- the kernels run none, and fp32_fma_rate_measured is the "
                              "FMA rate this device sustains without it (independent FMA chains, full grid); "
                              "achieved counts the evaluations the kernel runs: the reference's count minus "
                              "skipped_frac of it, (t0, c) cells that have two invalid traces and no result in any "
                              "arithmetic (achieved_reference_count counts all of them, as the reference's rate "
-                             "does); value is the reference's count per second; " + (
+                             "does); window_sample_rates: the device's rate of the search's inner step "
+                             "(interpolate one window sample, add it and its square) in FP32 as the kernels run "
+                             "it and in the mixed-precision forms a 16-bit store could use instead -- these are "
+                             "no faster, so the 16-bit stores restore their samples to FP32; "
+                             "value is the reference's count per second; " + (
                                  "windows come from LDS (two ds_read2_b32 per evaluation; a workgroup stages the "
                                  "ranges of as many of its traces as fit, once, and reads the rest from global "
                                  "memory): bound by VALU issue and the LDS data path, the staging latency partly "

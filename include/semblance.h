@@ -352,6 +352,16 @@ void sb_graph_destroy(sb_graph* gr);
  * published peak, which counts packed FP32 -- is the ceiling of their
  * arithmetic.  No reference counterpart (a measurement for the roofline). */
 int sb_device_fp32_fma_rate(int device, double* tflops);
+/* Interpolated window samples per second (the search's inner step: one
+ * interpolation, its window sum and its square) for four instruction forms,
+ * written out in every lane of a grid that fills every SIMD:
+ *   [0] FP32 samples: sub, fma, add, fma (what every store runs)
+ *   [1] FP16 sample and FP16 difference: v_fma_mix_f32, fma, fma
+ *   [2] FP16 sample pair, FP16 weights: v_dot2_f32_f16, fma, fma
+ *   [3] FP32 sample and FP32 difference: fma, fma, fma
+ * Forms 1 and 2 are the mixed-precision steps a 16-bit store could take
+ * instead of restoring its samples to FP32.  No reference counterpart. */
+int sb_device_window_rates(int device, double samples_per_s[4]);
 
 /* ------------------------------------------------------ synthetic data */
 /* Seeded synthetic gathers for the benchmark configurations (host, OpenMP).

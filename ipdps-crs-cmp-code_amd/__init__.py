@@ -102,7 +102,8 @@ EXPORTS = [
     "sb_last_timing", "sb_tune_set", "sb_tune_get", "sb_tune", "sb_synth", "sb_server_create", "sb_server_submit",
     "sb_server_wait", "sb_server_add_gather", "sb_server_remove_gather", "sb_server_stats", "sb_server_destroy",
     "sb_ctx_set_option", "sb_ctx_get_option", "sb_graph_create", "sb_graph_launch", "sb_graph_destroy",
-    "sb_search_streamed", "sb_cmp_run_noalloc", "sb_device_fp32_fma_rate", "sb_server_set_deadline",
+    "sb_search_streamed", "sb_cmp_run_noalloc", "sb_device_fp32_fma_rate", "sb_device_window_rates",
+    "sb_server_set_deadline",
 ]
 
 
@@ -183,6 +184,7 @@ def lib():
         L.sb_graph_destroy.restype = None
         L.sb_cmp_run_noalloc.argtypes = [P, P, C.POINTER(CmpParams), C.c_int, C.c_int, C.POINTER(DeviceArrays)]
         L.sb_device_fp32_fma_rate.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        L.sb_device_window_rates.argtypes = [C.c_int, C.POINTER(C.c_double)]
         L.sb_server_set_deadline.argtypes = [P, C.c_double]
         _lib = L
     return _lib
@@ -207,6 +209,17 @@ def fp32_fma_rate(device=0):
     t = C.c_double()
     check(lib().sb_device_fp32_fma_rate(device, C.byref(t)))
     return t.value
+
+
+WINDOW_FORMS = ("fp32_sub_fma", "fp16_fma_mix", "fp16_dot2", "fp32_pair_fma")
+
+
+def window_rates(device=0):
+    """sb_device_window_rates: interpolated window samples per second for the
+    four instruction forms of the search's inner step (WINDOW_FORMS)."""
+    r = (C.c_double * 4)()
+    check(lib().sb_device_window_rates(device, r))
+    return dict(zip(WINDOW_FORMS, list(r)))
 
 
 # ---------------------------------------------------------------- host side

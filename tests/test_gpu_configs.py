@@ -233,3 +233,15 @@ def test_device_fp32_fma_rate(sb):
     t = sb.fp32_fma_rate(0)
     print("sustained FP32 FMA rate: %.1f TFLOP/s" % t, flush=True)
     assert 10.0 < t < 200.0
+
+
+def test_device_window_rates(sb):
+    """Interpolated window samples per second for the four instruction forms
+    of the search's inner step (printed).  Every form runs and reports a
+    rate of the order of the FMA rate (one to four instructions per sample)."""
+    print("case: window arithmetic rate probe", flush=True)
+    r = sb.window_rates(0)
+    print("window samples/s: " + ", ".join("%s %.3g" % kv for kv in r.items()), flush=True)
+    fma = sb.fp32_fma_rate(0) * 1e12 / 2  # FMA per second
+    for k, v in r.items():
+        assert fma / 8 < v < fma * 2, (k, v, fma)
