@@ -71,3 +71,31 @@ def test_streamed_keeps_gather(sb, gpu_ctx):
         assert all(np.array_equal(other[k], _eager(sb, gpu_ctx, cmp_line, p2, sb.STORE_F32)[k]) for k in KEYS)
     finally:
         gpu_ctx.set_option(sb.OPT_STREAM_ROWS, 0)
+
+
+def test_streamed_after_queued_searches(sb, gpu_ctx):
+    """Device blocks freed while work that reads them is still queued on the
+    ctx stream are handed out again only after that work has run (the
+    allocator records an event on the stream at each free and the streamed
+    upload writes on its own copy stream): a streamed search issued right
+    after unsynchronised searches -- results never downloaded and destroyed,
+    work tables of several CDP ranges built and replaced, their gather
+    destroyed -- returns what the same streamed search returns on an idle
+    ctx, bit for bit."""
+    print("case: streamed after queued searches", flush=True)
+    cmp_line, _ = _lines(sb)
+    p = sb.cmp_params(1.98e-7, 1.77e-6, 64, 0.002)
+    args = (cmp_line["cdp_off"], cmp_line["geom"], cmp_line["samples"], cmp_line["dt_us"])
+    want = sb.search_streamed(gpu_ctx, p, *args)
+    gpu_ctx.sync()
+    off, geom, s = sb.synth(0, 160, (60, 120), 1501, 2000, 53, h_max=1500.0, h_jitter=5.0, nevents=6)
+    pb = sb.cmp_params(1.98e-7, 1.77e-6, 256, 0.002)
+    g = sb.Gather(gpu_ctx, off, geom, s, 2000)
+    for cb, ce in ((0, 160), (0, 40), (7, 151), (0, 160), (90, 100)):
+        r = sb.DeviceResult(gpu_ctx, ce - cb, 1501)
+        sb.run(gpu_ctx, g, pb, r, cdp_range=(cb, ce))
+        r.close()  # freed with its search still queued
+    g.close()
+    got = sb.search_streamed(gpu_ctx, p, *args)
+    for k in KEYS:
+        assert np.array_equal(got[k], want[k]), k
