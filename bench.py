@@ -88,6 +88,24 @@ GLOBAL_SHAPES = {0: (2, 4), 1: (1, 4), 2: (4, 2), 3: (2, 2)}
 LDS_SHAPES = {4: (1, 4, 8), 5: (1, 4, 8), 6: (1, 4, 8), 7: (1, 4, 8)}
 
 
+def effective_pchunk(res):
+    """Parameters per workgroup of a timed run: the tuned split, or for the
+    automatic split (0) the one the launch used, from its workgroup count
+    (CDPs x t0 tiles x parameter chunks, chunks of equal length)."""
+    v, pc = res["tune"][0], res["tune"][1]
+    if pc > 0:
+        return pc
+    if v in GLOBAL_SHAPES:
+        R, CC = GLOBAL_SHAPES[v]
+        unit = 4 * CC  # 4 waves per workgroup
+    else:
+        R, CC, NW = LDS_SHAPES[v]
+        unit = NW * CC
+    ntiles = -(-CFG["ns"] // (64 * R))
+    nchunks = max(1, res["blocks"] // (CFG["ncdp"] * ntiles))
+    return -(-(-(-CFG["nc"] // nchunks)) // unit) * unit
+
+
 def skipped_fraction(cdp_off, geom, variant, pchunk, nblocks):
     """Fraction of the reference's evaluations the CMP fast search never runs:
     a wave skips a parameter group when each of its (t0, c) already has two
@@ -595,7 +613,7 @@ def main():
     achieved = f32["tuples"] * FLOP_PER_TUPLE_CMP / (f32["search_ms"] * 1e-3) / 1e12
     fma_rate = sb.fp32_fma_rate(0)  # the device's sustained FP32 FMA rate without packed FP32
     win_rates = sb.window_rates(0)  # the inner step's instruction forms, window samples/s
-    skip = skipped_fraction(cdp_off, geom, f32["tune"][0], f32["tune"][1], f32["blocks"])
+    skip = skipped_fraction(cdp_off, geom, f32["tune"][0], effective_pchunk(f32), f32["blocks"])
     lds = bool(f32["lds_staged"])
     kernel = "k_search_lds" if lds else "k_search_fast"
     # HBM bytes per launch of the timed shape, from rocprofv3 FETCH_SIZE /
@@ -603,7 +621,7 @@ def main():
     # that shape was not profiled)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_%s_cmp_2000cdp_v%d_p%d.json" % (kernel, f32["tune"][0],
-                                                                                f32["tune"][1]))
+                                                                                effective_pchunk(f32)))
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
