@@ -85,7 +85,7 @@ def panel_latency(sb, ctx, cdp_off, geom, samples, n=100):
 # lane R, parameters per pass CC, waves per workgroup; variants >= 4 are
 # LDS-staged (k_search_lds: one group of CC parameters per wave and pass).
 GLOBAL_SHAPES = {0: (2, 4), 1: (1, 4), 2: (4, 2), 3: (2, 2)}
-LDS_SHAPES = {4: (1, 4, 8), 5: (1, 4, 8), 6: (1, 4, 8), 7: (1, 4, 8)}
+LDS_SHAPES = {4: (1, 4, 8), 5: (1, 4, 8), 6: (1, 4, 8), 7: (1, 4, 8), 8: (1, 4, 8), 9: (1, 4, 8), 10: (1, 4, 8)}
 
 
 def effective_pchunk(res):
@@ -589,14 +589,14 @@ def main():
             if args.variant >= 0:
                 ctx.tune_set(args.variant, args.pchunk)
             elif not args.no_tune:
-                v, pc, _ = sb.tune(ctx, g, prm)
+                v, pc, _ = sb.tune(ctx, g, prm, reps=4)
                 print("bench: tuned variant %d pchunk %d (%s)" % (v, pc, store), flush=True)
             wall, tk = timed_steps(sb, ctx, g, prm, args.steps, args.warmup)
             tuples = tk[0]["tuples"]
             search_ms = float(np.mean([t["search_ms"] for t in tk]))
             fin_ms = float(np.mean([t["finalize_ms"] for t in tk]))
             results[store] = dict(value=tuples * args.steps / wall, ms_per_step=wall / args.steps * 1e3,
-                                  search_ms=search_ms, finalize_ms=fin_ms, tuples=tuples, tune=ctx.tune_get()[:2],
+                                  search_ms=search_ms, finalize_ms=fin_ms, tuples=tuples, tune=(tk[0]["variant"], ctx.tune_get()[1]),
                                   blocks=tk[0]["grid_blocks"], lds_staged=tk[0]["lds_staged"])
             print("bench: %s %.4g evals/s, search %.3f ms, finalize %.3f ms" %
                   (store, results[store]["value"], search_ms, fin_ms), flush=True)
@@ -668,10 +668,11 @@ def main():
                              "it and in the mixed-precision forms a 16-bit store could use instead -- these are "
                              "no faster, so the 16-bit stores restore their samples to FP32; "
                              "value is the reference's count per second; " + (
-                                 "windows come from LDS (two ds_read2_b32 per evaluation; a workgroup stages the "
-                                 "ranges of as many of its traces as fit, once, and reads the rest from global "
-                                 "memory): bound by VALU issue and the LDS data path, the staging latency partly "
-                                 "hidden"
+                                 "windows come from LDS (a workgroup stages the ranges of as many of its traces "
+                                 "as fit, once, as FP32 samples or, in the paired shapes, each sample with its "
+                                 "forward difference so that a window sample is one FMA; the rest are read from "
+                                 "global memory): bound by VALU issue and the LDS data path, the staging latency "
+                                 "partly hidden"
                                  if lds else "the kernel is co-bound by the vector-memory address path (16-byte "
                                  "window loads) and VALU issue")},
     }

@@ -282,9 +282,12 @@ typedef struct {
 int sb_last_timing(const sb_ctx* ctx, sb_timing* t);
 
 /* Launch shape of the SB_MODE_FAST search on this ctx: variant indexes the
- * compiled (t0 rows per lane, parameters per pass) shapes; pchunk = parameters
- * per workgroup (0 = automatic).  sb_tune measures every shape on the given
- * gather and keeps the fastest (the project's autotuner). */
+ * compiled (t0 rows per lane, parameters per pass) shapes, -1 (a new ctx's
+ * setting) picks by store: the LDS-staged shape with (sample, difference)
+ * slabs for FP32, the one with sample slabs for 16-bit stores; pchunk =
+ * parameters per workgroup (0 = automatic).  sb_tune measures every shape on
+ * the given gather and keeps the fastest (the project's autotuner); sb_last_timing
+ * reports the shape that ran. */
 int sb_tune_set(sb_ctx* ctx, int variant, int pchunk);
 int sb_tune_get(const sb_ctx* ctx, int* variant, int* pchunk, int* nvariants);
 int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int reps);

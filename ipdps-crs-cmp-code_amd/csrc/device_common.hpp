@@ -125,7 +125,7 @@ __device__ __forceinline__ Win<N> load_win(__amdgpu_buffer_rsrc_t rs, unsigned o
       r.v[i + 1] = q.y;
       i += 2;
     }
-    if constexpr (N % 2 == 1) r.v[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u * i, 0, 0);
+    if constexpr (N % 2 == 1) r.v[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u * i, 0, 0));
   } else {
     const WinH<N> h = load_win_h<N>(rs, off, sh);
 #pragma unroll

@@ -14,6 +14,7 @@ struct FastVariant {
   int lds_kb;  // 0: windows from global memory (k_search_fast); else KiB of
                // LDS slab per block (k_search_lds, CMP only)
   int waves;   // waves per block (a block's waves split its parameters)
+  int pairs;   // k_search_lds: slabs of (sample, forward difference) pairs, 8 bytes a sample
 };
 
 // LDS-staged CMP search: at most this many dynamic LDS bytes per block and
@@ -21,12 +22,16 @@ struct FastVariant {
 constexpr int kLdsMaxBytes = 64 * 1024;
 constexpr int kLdsMaxTraces = 2048;
 // Byte offset of the slabs in an LDS-staged block for a CDP of S traces
-// (-1: no such shape); the slabs hold FP32 samples.
+// (-1: no such shape); the slabs hold FP32 samples (or FP32 pairs).
 int lds_slab_offset(const FastVariant& v, int S);
 extern const FastVariant kFastVariants[];
 extern const int kNumFastVariants;
-// Shape of an untuned ctx: LDS-staged, 8 waves, 52 KiB (CMP; searches it
-// cannot stage run kGlobalFallbackVariant).
+// Shapes of an untuned ctx (CMP; searches they cannot stage run
+// kGlobalFallbackVariant): LDS-staged, 8 waves, 52 KiB, with (sample,
+// difference) slabs for FP32 (one FMA per window sample) and sample slabs for
+// 16-bit stores (whose staging would convert twice as many values into half
+// as many staged traces).
+constexpr int kDefaultFastVariantF32 = 9;
 constexpr int kDefaultFastVariant = 5;
 // Global-memory shape an LDS-staged variant falls back to (R = 1, CC = 4:
 // the fastest global shape on the CRS grid and on the CMP line).

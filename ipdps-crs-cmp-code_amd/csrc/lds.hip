@@ -52,6 +52,11 @@ __device__ __forceinline__ unsigned lshl1_add(int x, unsigned y) {
   asm("v_lshl_add_u32 %0, %1, 1, %2" : "=v"(d) : "v"(x), "s"(y));
   return d;
 }
+__device__ __forceinline__ unsigned lshl3_add(int x, unsigned y) {
+  unsigned d;
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(d) : "v"(x), "v"(y));
+  return d;
+}
 __device__ __forceinline__ unsigned lshl4_add(int x, unsigned y) {
   unsigned d;
   asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(d) : "v"(x), "s"(y));
@@ -80,10 +85,11 @@ struct TraceLds {
   unsigned base;
 };
 
-// One evaluation in flight: its window and interpolation fraction.
+// One evaluation in flight: its window (W + 1 samples, or W (sample,
+// difference) pairs from a paired slab) and interpolation fraction.
 template <int W>
 struct Ev {
-  float v[W + 1];
+  float v[2 * W > W + 1 ? 2 * W : W + 1];
   float x;
   bool ok;
 };
@@ -101,12 +107,15 @@ struct LdsLayout {
   static __host__ __device__ constexpr int slab(int S) { return kTr + ((8 * S + 4 * (2 * S + 1) + 15) & ~15); }
 };
 
-template <int W, int R, int CC, int STORE, int NW>
+template <int W, int R, int CC, int STORE, int NW, bool PR>
 __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   using L = LdsLayout<NW>;
   constexpr int NT = 64 * NW;
   constexpr int RP = NW * CC;  // parameters per pass of the block's waves
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
+  // PR: the slabs hold each sample with its forward difference (8 bytes a
+  // sample, 32 a quad), so a window sample is one FMA; else 4 and 16.
+  constexpr int QB = PR ? 32 : 16;
   extern __shared__ __align__(16) unsigned char dyn[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar trace indices
@@ -123,7 +132,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   int* s_pre = (int*)(dyn + L::kTr + 8 * S);
   int* s_lo = s_pre + (S + 1);
   const int smp = L::slab(S);
-  const int capQ = (a.lds_bytes - smp) >> 4;  // quads the slabs can hold
+  const int capQ = (a.lds_bytes - smp) / QB;  // quads the slabs can hold
 
   if (tid < 16) ((uint32_t*)dyn)[tid] = 0u;
   for (int k = tid; k < S; k += NT) s_tr[k] = TraceLds{sl[k].h, 0u};  // h read back below by the same thread
@@ -216,10 +225,20 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         ev[r][ci].ok = ok;
         ev[r][ci].x = __builtin_amdgcn_fractf(ts);
         if constexpr (FROM_LDS) {
-          unsigned addr = lshl2_add(it, t.base);
+          unsigned addr = PR ? lshl3_add(it, t.base) : lshl2_add(it, t.base);
           if constexpr (GATE != 1) addr = ok ? addr : 0u;
+          if constexpr (PR) {  // 8-byte aligned (sample, difference) pairs
+            typedef float f2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-          for (int j = 0; j <= W; j++) ev[r][ci].v[j] = lds_at<float>(addr + 4u * j);
+            for (int j = 0; j < W; j++) {
+              const f2 q = lds_at<f2>(addr + 8u * j);
+              ev[r][ci].v[2 * j] = q.x;
+              ev[r][ci].v[2 * j + 1] = q.y;
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j <= W; j++) ev[r][ci].v[j] = lds_at<float>(addr + 4u * j);
+          }
         } else {
           unsigned off, shf = 0u;
           if constexpr (STORE == SB_STORE_F32) {
@@ -236,9 +255,11 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       }
     }
   };
-  // k_search_fast's window arithmetic, operation for operation.
-  auto consume = [&](const Ev<W>(&ev)[R][CC], auto gate_tag) {
+  // k_search_fast's window arithmetic, operation for operation (a paired
+  // slab holds the difference the other windows form: the same bits).
+  auto consume = [&](const Ev<W>(&ev)[R][CC], auto gate_tag, auto lds_tag) {
     constexpr int GATE = decltype(gate_tag)::value;
+    constexpr bool PAIRED = PR && decltype(lds_tag)::value;
 #pragma unroll
     for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -248,7 +269,8 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         const float x = e.x;
 #pragma unroll
         for (int j = 0; j < W; j++) {
-          const float v = fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
+          const float v = PAIRED ? fmaf(e.v[2 * j + 1], x, e.v[2 * j])
+                                 : fmaf(__fsub_rn(e.v[j + 1], e.v[j]), x, e.v[j]);
           num[r][ci][j] = __fadd_rn(num[r][ci][j], v);
           den[r][ci] = fmaf(v, v, den[r][ci]);
         }
@@ -277,7 +299,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       const TraceRec t = tr(k, FROM_LDS);
       Ev<W> A[R][CC];
       issue(t, FROM_LDS ? 0 : slc[k].row, A, gate_tag, lds_tag);
-      consume(A, gate_tag);
+      consume(A, gate_tag, lds_tag);
     }
   };
   // All traces of the group, in slot order: traces [0, K) from their staged
@@ -415,8 +437,8 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           qlo[g] = __builtin_amdgcn_readfirstlane(s_lo[k]);
           nq[g] = kg + g < K ? __builtin_amdgcn_readfirstlane(s_pre[k + 1]) - qb : 0;
           nqm = max(nqm, nq[g]);
-          dq[g] = (unsigned)(smp + 16 * qb);
-          if (lane == 0 && kg + g < K) s_tr[k].base = (unsigned)(smp + 16 * (qb - qlo[g]) - 4 * a.tau);
+          dq[g] = (unsigned)(smp + QB * qb);
+          if (lane == 0 && kg + g < K) s_tr[k].base = (unsigned)(smp + QB * (qb - qlo[g]) - (QB / 4) * a.tau);
           const int row = slc[k].row;
           sc[g] = STORE == SB_STORE_F32 ? 1.f : slc[k].scale;
           rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
@@ -424,9 +446,23 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         }
         for (int u = lane; u < nqm; u += 64) {
           f4 v[G];
+          float nx[G];  // PR: the sample after the quad (its last difference)
 #pragma unroll
           for (int g = 0; g < G; g++) {
             const unsigned o = (unsigned)(qlo[g] + u) * (4 * ESZ);
+            if constexpr (PR) {
+              if constexpr (STORE == SB_STORE_F32) {
+                nx[g] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs[g], o + 16u, 0, 0));
+              } else {
+                const unsigned n2 = __builtin_amdgcn_raw_buffer_load_b32(rs[g], o + 8u, 0, 0);
+                if constexpr (STORE == SB_STORE_F16) {
+                  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+                  nx[g] = (float)__builtin_bit_cast(h2v, n2).x * sc[g];
+                } else {
+                  nx[g] = __uint_as_float(n2 << 16) * sc[g];
+                }
+              }
+            }
             if constexpr (STORE == SB_STORE_F32) {
               v[g] = __builtin_amdgcn_raw_buffer_load_b128(rs[g], o, 0, 0);
             } else {
@@ -445,8 +481,19 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
             }
           }
 #pragma unroll
-          for (int g = 0; g < G; g++)
-            if (u < nq[g]) *(__attribute__((address_space(3))) f4*)(size_t)(dq[g] + 16u * u) = v[g];
+          for (int g = 0; g < G; g++) {
+            if (u >= nq[g]) continue;
+            typedef __attribute__((address_space(3))) f4 LF4;
+            if constexpr (PR) {
+              const f4 q = v[g];
+              const float d0 = __fsub_rn(q.y, q.x), d1 = __fsub_rn(q.z, q.y), d2 = __fsub_rn(q.w, q.z),
+                          d3 = __fsub_rn(nx[g], q.w);
+              *(LF4*)(size_t)(dq[g] + 32u * u) = f4{q.x, d0, q.y, d1};
+              *(LF4*)(size_t)(dq[g] + 32u * u + 16u) = f4{q.z, d2, q.w, d3};
+            } else {
+              *(LF4*)(size_t)(dq[g] + 16u * u) = v[g];
+            }
+          }
         }
       }
       __syncthreads();
@@ -491,25 +538,25 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
 }
 
 // ------------------------------------------------------------- launchers
-template <int W, int R, int CC, int NW>
+template <int W, int R, int CC, int NW, bool PR>
 static hipError_t lds_store(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
   const size_t sh = (size_t)a.lds_bytes;
   if (store == SB_STORE_F32)
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F32, NW>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F32, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   else if (store == SB_STORE_F16)
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F16, NW>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F16, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   else
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_BF16, NW>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_BF16, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   return hipGetLastError();
 }
 
-template <int R, int CC, int NW>
+template <int R, int CC, int NW, bool PR = false>
 static hipError_t lds_w(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
   switch (a.w) {
-    case 1: return lds_store<1, R, CC, NW>(a, store, nblocks, st);
-    case 3: return lds_store<3, R, CC, NW>(a, store, nblocks, st);
-    case 5: return lds_store<5, R, CC, NW>(a, store, nblocks, st);
-    case 7: return lds_store<7, R, CC, NW>(a, store, nblocks, st);
+    case 1: return lds_store<1, R, CC, NW, PR>(a, store, nblocks, st);
+    case 3: return lds_store<3, R, CC, NW, PR>(a, store, nblocks, st);
+    case 5: return lds_store<5, R, CC, NW, PR>(a, store, nblocks, st);
+    case 7: return lds_store<7, R, CC, NW, PR>(a, store, nblocks, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -527,7 +574,8 @@ hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& 
   if (nblocks <= 0) return hipSuccess;
   if (a.method != SB_METHOD_CMP || a.lds_bytes <= 0 || a.lds_bytes > kLdsMaxBytes) return hipErrorInvalidValue;
   if (a.lds_bytes < 64 + v.waves * 64 * v.R * 8) return hipErrorInvalidValue;  // the keys at the end
-  if (v.R == 1 && v.CC == 4 && v.waves == 8) return lds_w<1, 4, 8>(a, store, nblocks, st);
+  if (v.R == 1 && v.CC == 4 && v.waves == 8)
+    return v.pairs ? lds_w<1, 4, 8, true>(a, store, nblocks, st) : lds_w<1, 4, 8>(a, store, nblocks, st);
   if (v.R == 1 && v.CC == 4 && v.waves == 16) return lds_w<1, 4, 16>(a, store, nblocks, st);
   if (v.R == 2 && v.CC == 2 && v.waves == 8) return lds_w<2, 2, 8>(a, store, nblocks, st);
   if (v.R == 2 && v.CC == 2 && v.waves == 16) return lds_w<2, 2, 16>(a, store, nblocks, st);

@@ -723,7 +723,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   // parameters per workgroup.
   int family, ntiles, npar_w, pchunk;
   if (mode == SB_MODE_FAST) {
-    variant = ctx->fast_variant;
+    variant = ctx->fast_variant >= 0 ? ctx->fast_variant
+              : store == SB_STORE_F32 ? kDefaultFastVariantF32
+                                      : kDefaultFastVariant;
     v = kFastVariants[variant];
     if (v.lds_kb > 0 && a.method == SB_METHOD_CMP && lds_fits(g, a, v, cb, ce)) {
       lds = true;

@@ -33,7 +33,7 @@ struct sb_ctx {
   bool timing_graph = false;  // the pending timing is a graph replay
   int num_cus = 256;
   // launch shape of the fast search (autotuned or set by sb_tune_set)
-  int fast_variant = sb::kDefaultFastVariant;
+  int fast_variant = -1;  // -1: by store (kDefaultFastVariantF32 / kDefaultFastVariant)
   int pchunk_override = 0;
   std::map<std::string, std::pair<int, int>> tune_cache;
   // reusable work table / queue counters

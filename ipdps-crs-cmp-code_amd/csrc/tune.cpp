@@ -19,7 +19,7 @@
 using namespace sb;
 
 extern "C" int sb_tune_set(sb_ctx* ctx, int variant, int pchunk) {
-  if (!ctx || variant < 0 || variant >= kNumFastVariants || pchunk < 0) return fail(SB_ERR_INVALID, "bad tuning");
+  if (!ctx || variant < -1 || variant >= kNumFastVariants || pchunk < 0) return fail(SB_ERR_INVALID, "bad tuning");
   ctx->fast_variant = variant;
   ctx->pchunk_override = pchunk;
   return SB_OK;

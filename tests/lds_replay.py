@@ -9,7 +9,8 @@ Restates the kernel's arithmetic in FP32 (numpy's square root is correctly
 rounded; the hardware's is within 1 ulp, which moves a range end by at most
 one sample and the counts negligibly), and the LDS layout of an 8-wave
 workgroup: 64 zero bytes, the scan scratch, TraceLds[S], s_pre[S + 1],
-s_lo[S], then 16-byte slabs.  Test infrastructure only.
+s_lo[S], then the slabs: 16 bytes per quad of samples, or 32 for the shapes
+that stage (sample, difference) pairs.  Test infrastructure only.
 """
 import numpy as np
 
@@ -36,7 +37,7 @@ def slot_h(cdp_off, geom, dt_us):
     return [np.sort(h[cdp_off[i]:cdp_off[i + 1]], kind="stable") for i in range(len(cdp_off) - 1)]
 
 
-def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w_compiled=None):
+def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w_compiled=None, quad_bytes=16):
     """Counts of work items by staging case: dict(full, partial, none, dead)."""
     dt = f32(f32(dt_us) / f32(1e6))
     tau = max(int(f32(tau_s) * (f32(1.0) / dt)), 0)
@@ -49,7 +50,7 @@ def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w
     out = dict(full=0, partial=0, none=0, dead=0)
     for hs in slot_h(cdp_off, geom, dt_us):
         S = len(hs)
-        capQ = (lds_kb * 1024 - slab_offset(S)) >> 4
+        capQ = (lds_kb * 1024 - slab_offset(S)) // quad_bytes
         h2 = hs[S - 2] if S >= 2 else f32(0)
         for t0min in range(0, ns, 64):
             sqlo = f32(f32(t0min) * f32(t0min))

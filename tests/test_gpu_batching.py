@@ -48,7 +48,7 @@ def test_persistent_grid_equals_per_item_launch(sb, gpu_ctx, store):
                 assert all(np.array_equal(out2[k], ref[k][1:5]) for k in KEYS), v
     finally:
         gpu_ctx.set_option(sb.OPT_PERSISTENT, 0)
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
         gc.close()
 

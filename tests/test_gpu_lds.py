@@ -45,7 +45,7 @@ def test_lds_equals_global(sb, gpu_ctx, store, tau, dt_us):
                     for k in KEYS:
                         assert np.array_equal(out[k], ref[k]), (v, pc, c0, k)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -68,7 +68,7 @@ def test_lds_ranges_beyond_lds(sb, gpu_ctx, store):
                 for k in KEYS:
                     assert np.array_equal(out[k], ref[k]), (v, pc, k)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -92,7 +92,7 @@ def test_lds_partial_staging(sb, gpu_ctx, store):
                 for k in KEYS:
                     assert np.array_equal(out[k], ref[k]), (v, pc, k)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -115,7 +115,7 @@ def test_lds_far_offsets_stage_nothing(sb, gpu_ctx):
             for k in KEYS:
                 assert np.array_equal(out[k], ref[k]), (v, k)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -133,7 +133,7 @@ def test_lds_falls_back_when_it_cannot_stage(sb, gpu_ctx):
         for k in KEYS:
             assert np.array_equal(out[k], ref[k]), k
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -143,12 +143,12 @@ def test_lds_range_search(sb, gpu_ctx):
     g = sb.Gather(gpu_ctx, off, geom, s, 2000, store=sb.STORE_F16)
     try:
         prm = sb.cmp_params(1.98e-7, 1.77e-6, 64, 0.002)
-        gpu_ctx.tune_set(5, 0)
+        gpu_ctx.tune_set(-1, 0)
         full = sb.search(gpu_ctx, g, prm)
         part = sb.search(gpu_ctx, g, prm, cdp_range=(2, 6))
         assert gpu_ctx.timing()["lds_staged"] == 1
         for k in KEYS:
             assert np.array_equal(part[k], full[k][2:6]), k
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()

@@ -44,7 +44,7 @@ def test_cmp_16bit_store_equals_f32_on_restored_samples(sb, gpu_ctx, store, vari
         b = sb.search(gpu_ctx, g32, prm)
         g32.close()
     finally:
-        gpu_ctx.tune_set(5, 0)
+        gpu_ctx.tune_set(-1, 0)
     assert np.array_equal(a["index"], b["index"])
     assert (a["index"] >= 0).mean() > 0.3
 

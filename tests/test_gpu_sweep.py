@@ -95,7 +95,7 @@ def test_cmp_sweep(sb, orc, gpu_ctx, i):
         fast = sb.search(gpu_ctx, g, sb.cmp_params(c0, c1, nc, tau))
     finally:
         gpu_ctx.set_option(sb.OPT_PERSISTENT, 0)
-        gpu_ctx.tune_set(5, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
     ref = orc.cmp(off, geom, s, dt_us, c0, c1, nc, tau, panel=True)
     for k in KEYS_CMP:
@@ -120,7 +120,7 @@ def test_crs_sweep(sb, orc, gpu_ctx, i):
         gpu_ctx.tune_set(variant, 0)
         fast = sb.search(gpu_ctx, g, sb.crs_params(*grid))
     finally:
-        gpu_ctx.tune_set(5, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
     ref = orc.crs(off, geom, s, dt_us, nbr_off, nbr, *grid, panel=True)
     for k in KEYS_CRS:

@@ -45,7 +45,7 @@ def test_launch_shapes_bit_identical(sb, gpu_ctx, store, tau):
             for k in KEYS:
                 assert np.array_equal(o[k], ref[k]), (v, k)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g.close()
 
 
@@ -70,7 +70,7 @@ def test_picks_independent_of_trace_order(sb, gpu_ctx, store):
             ok = a["index"] >= 0
             assert np.allclose(a["coher"][ok], b["coher"][ok], rtol=1e-5, atol=1e-6)
     finally:
-        gpu_ctx.tune_set(0, 0)
+        gpu_ctx.tune_set(-1, 0)
         g1.close()
         g2.close()
 

@@ -15,7 +15,9 @@ so each such window must lie inside the staged quads; in the general path
 only valid windows (0 <= e < ns - w and t^2 < hi2) are read.  This test
 evaluates both sides in FP32 with the square roots pushed 1 ulp the wrong
 way (the bound's inward, the lanes' outward) on random lines, parameter
-grids and both grid directions, including the last, partial tile."""
+grids and both grid directions, including the last, partial tile.  Shapes
+that stage (sample, difference) pairs stage the same samples and read the W
+pairs of samples e .. e + W - 1, a subset of the same window."""
 import numpy as np
 
 f32 = np.float32
@@ -98,16 +100,19 @@ def test_staging_cases_reached_by_the_gpu_tests(sb):
     import lds_replay as lr
     seen = dict(full=0, partial=0, none=0, dead=0)
     # test_lds_partial_staging: headline geometry, 128 and 256 parameters per workgroup, 40-64 KiB
+    # of sample slabs and 40-64 KiB of (sample, difference) slabs
     off, geom, _ = sb.synth(0, 5, (60, 120), 1501, 2000, 31, h_max=1500.0, h_jitter=5.0, nevents=8)
-    for kb in (40, 48, 52, 64):
+    paired = dict(full=0, partial=0, none=0, dead=0)
+    for kb, qb in ((40, 16), (48, 16), (52, 16), (64, 16), (64, 32), (52, 32), (40, 32)):
         for pc in (128, 256):
-            c = lr.staging_cases(off, geom, 2000, 1501, 0.002, 1.98e-7, 1.77e-6, 256, pc, kb)
+            c = lr.staging_cases(off, geom, 2000, 1501, 0.002, 1.98e-7, 1.77e-6, 256, pc, kb, quad_bytes=qb)
             for k in seen:
-                seen[k] += c[k]
+                (seen if qb == 16 else paired)[k] += c[k]
     part = dict(seen)
     # test_lds_far_offsets_stage_nothing
     off, geom, _ = lr.far_offset_line()
     c = lr.staging_cases(off, geom, 500, 20001, 0.001, 1e-8, 5e-6, 16, 16, 40)
-    print("partial-staging gathers: %s; far-offset gather: %s" % (part, c))
+    print("partial-staging gathers: %s, paired slabs %s; far-offset gather: %s" % (part, paired, c))
     assert part["full"] > 0 and part["partial"] > 0
+    assert paired["full"] > 0 and paired["partial"] > 0
     assert c["none"] > 0
