@@ -85,7 +85,7 @@ def panel_latency(sb, ctx, cdp_off, geom, samples, n=100):
 # lane R, parameters per pass CC, waves per workgroup; variants >= 4 are
 # LDS-staged (k_search_lds: one group of CC parameters per wave and pass).
 GLOBAL_SHAPES = {0: (2, 4), 1: (1, 4), 2: (4, 2), 3: (2, 2)}
-LDS_SHAPES = {4: (1, 4, 8), 5: (1, 4, 8), 6: (1, 4, 8), 7: (1, 4, 8), 8: (1, 4, 8), 9: (1, 4, 8), 10: (1, 4, 8)}
+LDS_SHAPES = {v: (1, 4, 8) for v in range(4, 13)}
 
 
 def effective_pchunk(res):

@@ -286,8 +286,9 @@ int sb_last_timing(const sb_ctx* ctx, sb_timing* t);
  * setting) picks by store: the LDS-staged shape with (sample, difference)
  * slabs for FP32, the one with sample slabs for 16-bit stores; pchunk =
  * parameters per workgroup (0 = automatic).  sb_tune measures every shape on
- * the given gather and keeps the fastest (the project's autotuner); sb_last_timing
- * reports the shape that ran. */
+ * the given gather (reps timed runs each), re-times the three fastest in
+ * 2 reps + 1 interleaved rounds and keeps the one with the lowest median
+ * (the project's autotuner); sb_last_timing reports the shape that ran. */
 int sb_tune_set(sb_ctx* ctx, int variant, int pchunk);
 int sb_tune_get(const sb_ctx* ctx, int* variant, int* pchunk, int* nvariants);
 int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int reps);

@@ -81,6 +81,7 @@ struct SearchArgs {
   int nwork;
   unsigned int* queue;       // persistent-kernel counters (8, one per XCD) or null
   int lds_bytes;             // dynamic LDS per block of the LDS-staged search
+  int pair_share;            // mixed slabs: sixteenths of the slab bytes the paired slabs may take
 };
 
 // DE settings.

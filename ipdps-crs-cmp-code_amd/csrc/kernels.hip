@@ -1301,7 +1301,8 @@ bool fast_supported(int w) { return w == 1 || w == 3 || w == 5 || w == 7; }
 
 const FastVariant kFastVariants[] = {{2, 4, 0, 4, 0},  {1, 4, 0, 4, 0},  {4, 2, 0, 4, 0},  {2, 2, 0, 4, 0},
                                      {1, 4, 48, 8, 0}, {1, 4, 52, 8, 0}, {1, 4, 64, 8, 0}, {1, 4, 40, 8, 0},
-                                     {1, 4, 64, 8, 1}, {1, 4, 52, 8, 1}, {1, 4, 40, 8, 1}};
+                                     {1, 4, 64, 8, 1}, {1, 4, 52, 8, 1}, {1, 4, 40, 8, 1}, {1, 4, 52, 8, 2, 16},
+                                     {1, 4, 52, 8, 2, 8}};
 const int kNumFastVariants = sizeof(kFastVariants) / sizeof(kFastVariants[0]);
 
 hipError_t launch_search_fast(const SearchArgs& a, int store, int variant, int nblocks, int num_cus, hipStream_t st) {

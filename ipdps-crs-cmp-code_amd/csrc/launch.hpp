@@ -14,7 +14,10 @@ struct FastVariant {
   int lds_kb;  // 0: windows from global memory (k_search_fast); else KiB of
                // LDS slab per block (k_search_lds, CMP only)
   int waves;   // waves per block (a block's waves split its parameters)
-  int pairs;   // k_search_lds: slabs of (sample, forward difference) pairs, 8 bytes a sample
+  int pairs;   // k_search_lds slabs: 0 samples (4 bytes a sample); 1 (sample, forward difference)
+               // pairs (8 bytes a sample); 2 pairs for the nearest traces, samples after them
+  int share;   // pairs == 2: sixteenths of the slab bytes the pairs may take when the samples of
+               // every trace do not fit
 };
 
 // LDS-staged CMP search: at most this many dynamic LDS bytes per block and

@@ -22,6 +22,18 @@
 // the zeros at address 0) and the windows of the other traces from global
 // memory as k_search_fast does.
 //
+// Paired slabs hold every sample with its forward difference (8 bytes a
+// sample, the difference the trace loop would form), so a window sample is
+// one FMA; they hold half the samples per byte.  PM selects the slabs:
+// 0 samples only; 1 pairs only (the first K traces that fit); 2 pairs for
+// the first Kp traces and samples for traces Kp .. K-1 -- every trace as
+// pairs when that fits, else every trace staged with as many pairs as the
+// rest of the slab holds when the samples fit, else pairs in at most
+// pair_share/16 of the slab bytes and samples after them.  A window read
+// from a paired slab costs 24 LDS bytes at w = 3 against 16 from a sample
+// slab and 3 fewer VALU: mixing the two balances the loop between the LDS
+// data path and VALU issue.
+//
 // Every rounding step matches k_search_fast, in the same trace order, with
 // the same general/prefix split, last-trace gate and second-to-last-trace
 // validity, so both kernels produce identical keys.  Every store runs the
@@ -107,15 +119,13 @@ struct LdsLayout {
   static __host__ __device__ constexpr int slab(int S) { return kTr + ((8 * S + 4 * (2 * S + 1) + 15) & ~15); }
 };
 
-template <int W, int R, int CC, int STORE, int NW, bool PR>
+template <int W, int R, int CC, int STORE, int NW, int PM>
 __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   using L = LdsLayout<NW>;
   constexpr int NT = 64 * NW;
   constexpr int RP = NW * CC;  // parameters per pass of the block's waves
   constexpr int ESZ = STORE == SB_STORE_F32 ? 4 : 2;
-  // PR: the slabs hold each sample with its forward difference (8 bytes a
-  // sample, 32 a quad), so a window sample is one FMA; else 4 and 16.
-  constexpr int QB = PR ? 32 : 16;
+  static_assert(PM >= 0 && PM <= 2, "slab layout");
   extern __shared__ __align__(16) unsigned char dyn[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar trace indices
@@ -132,7 +142,9 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   int* s_pre = (int*)(dyn + L::kTr + 8 * S);
   int* s_lo = s_pre + (S + 1);
   const int smp = L::slab(S);
-  const int capQ = (a.lds_bytes - smp) / QB;  // quads the slabs can hold
+  // Slab capacity in 16-byte units: a quad of samples takes one, a quad of
+  // (sample, difference) pairs two.
+  const int cap16 = (a.lds_bytes - smp) >> 4;
 
   if (tid < 16) ((uint32_t*)dyn)[tid] = 0u;
   for (int k = tid; k < S; k += NT) s_tr[k] = TraceLds{sl[k].h, 0u};  // h read back below by the same thread
@@ -197,14 +209,17 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       }
   };
 
-  // Reads the windows of trace `t` (LDS: `t.base`; global memory: `row`) for
-  // every (row, parameter).  GATE 0: window and t^2 test (general path);
-  // 1: none; 2: the t^2 bound (last trace).  Both sources give the same
-  // values: the stored samples converted exactly to FP32, zeros for gated
-  // windows.
-  auto issue = [&](const TraceRec& t, int row, Ev<W>(&ev)[R][CC], auto gate_tag, auto lds_tag) {
+  // Window sources of a trace: global memory, a sample slab, a paired slab.
+  constexpr int kGlobal = 0, kSamples = 1, kPairs = 2;
+  // Reads the windows of trace `t` (slabs: `t.base`; global memory: `row`)
+  // for every (row, parameter).  GATE 0: window and t^2 test (general path);
+  // 1: none; 2: the t^2 bound (last trace).  Every source gives the same
+  // values: the stored samples converted exactly to FP32 (a paired slab also
+  // the differences the consumer would form), zeros for gated windows.
+  auto issue = [&](const TraceRec& t, int row, Ev<W>(&ev)[R][CC], auto gate_tag, auto src_tag) {
     constexpr int GATE = decltype(gate_tag)::value;
-    constexpr bool FROM_LDS = decltype(lds_tag)::value;
+    constexpr int SRC = decltype(src_tag)::value;
+    constexpr bool FROM_LDS = SRC != kGlobal;
     __amdgpu_buffer_rsrc_t rs;
     if constexpr (!FROM_LDS) {
       const int lead = STORE == SB_STORE_F32 ? 4 * a.tau : 2 * tau_even;
@@ -225,9 +240,9 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         ev[r][ci].ok = ok;
         ev[r][ci].x = __builtin_amdgcn_fractf(ts);
         if constexpr (FROM_LDS) {
-          unsigned addr = PR ? lshl3_add(it, t.base) : lshl2_add(it, t.base);
+          unsigned addr = SRC == kPairs ? lshl3_add(it, t.base) : lshl2_add(it, t.base);
           if constexpr (GATE != 1) addr = ok ? addr : 0u;
-          if constexpr (PR) {  // 8-byte aligned (sample, difference) pairs
+          if constexpr (SRC == kPairs) {  // 8-byte aligned (sample, difference) pairs
             typedef float f2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int j = 0; j < W; j++) {
@@ -257,9 +272,9 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   };
   // k_search_fast's window arithmetic, operation for operation (a paired
   // slab holds the difference the other windows form: the same bits).
-  auto consume = [&](const Ev<W>(&ev)[R][CC], auto gate_tag, auto lds_tag) {
+  auto consume = [&](const Ev<W>(&ev)[R][CC], auto gate_tag, auto src_tag) {
     constexpr int GATE = decltype(gate_tag)::value;
-    constexpr bool PAIRED = PR && decltype(lds_tag)::value;
+    constexpr bool PAIRED = decltype(src_tag)::value == kPairs;
 #pragma unroll
     for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -293,32 +308,43 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   // Traces [kb, ke): each trace's windows are read and accumulated in turn
   // (the waves of the CU hide the latency); global-memory windows take the
   // row from the slot table (scalar: uniform row addressing).
-  auto run = [&](int kb, int ke, auto gate_tag, auto lds_tag) {
-    constexpr bool FROM_LDS = decltype(lds_tag)::value;
+  auto run = [&](int kb, int ke, auto gate_tag, auto src_tag) {
+    constexpr bool FROM_LDS = decltype(src_tag)::value != kGlobal;
     for (int k = kb; k < ke; k++) {
       const TraceRec t = tr(k, FROM_LDS);
       Ev<W> A[R][CC];
-      issue(t, FROM_LDS ? 0 : slc[k].row, A, gate_tag, lds_tag);
-      consume(A, gate_tag, lds_tag);
+      issue(t, FROM_LDS ? 0 : slc[k].row, A, gate_tag, src_tag);
+      consume(A, gate_tag, src_tag);
     }
   };
-  // All traces of the group, in slot order: traces [0, K) from their staged
-  // ranges, [K, S) from global memory.
-  auto accumulate = [&](int K) {
+  // All traces of the group, in slot order: traces [0, Kp) from paired
+  // slabs, [Kp, K) from sample slabs, [K, S) from global memory.
+  auto accumulate = [&](int Kp, int K) {
     using G0 = std::integral_constant<int, 0>;
     using G1 = std::integral_constant<int, 1>;
     using G2 = std::integral_constant<int, 2>;
+    using SG = std::integral_constant<int, kGlobal>;
+    using SS = std::integral_constant<int, kSamples>;
+    using SP = std::integral_constant<int, kPairs>;
     if (prefix) {
-      const int kp = min(K, S - 1);
-      if (kp > 0) run(0, kp, G1{}, std::true_type{});
-      if (S - 1 > kp) run(kp, S - 1, G1{}, std::false_type{});
-      if (K == S)
-        run(S - 1, S, G2{}, std::true_type{});
+      const int kp = min(Kp, S - 1), ks = min(K, S - 1);
+      if constexpr (PM != 0)
+        if (kp > 0) run(0, kp, G1{}, SP{});
+      if constexpr (PM != 1)
+        if (ks > kp) run(kp, ks, G1{}, SS{});
+      if (S - 1 > ks) run(ks, S - 1, G1{}, SG{});
+      if (PM != 0 && Kp == S)
+        run(S - 1, S, G2{}, SP{});
+      else if (PM != 1 && K == S)
+        run(S - 1, S, G2{}, SS{});
       else
-        run(S - 1, S, G2{}, std::false_type{});
+        run(S - 1, S, G2{}, SG{});
     } else {
-      if (K > 0) run(0, K, G0{}, std::true_type{});
-      if (S > K) run(K, S, G0{}, std::false_type{});
+      if constexpr (PM != 0)
+        if (Kp > 0) run(0, Kp, G0{}, SP{});
+      if constexpr (PM != 1)
+        if (K > Kp) run(Kp, K, G0{}, SS{});
+      if (S > K) run(K, S, G0{}, SG{});
     }
   };
   // The group's results into the running keys.
@@ -361,7 +387,8 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
   // out-of-range second-to-last t^2 at the first t0 (the same float values
   // as that lane's); block-uniform.
   const bool dead = prefix && S >= 2 && !(fmaf(cmin, h2, sqlo) < a.hi2);
-  int K = 0;  // traces staged in LDS (the first K in slot order)
+  int K = 0;   // traces staged in LDS (the first K in slot order)
+  int Kp = 0;  // of which the first Kp as (sample, difference) pairs
   if (!dead) {
     // ---- each trace's range of window starts: e = floor(sqrt(T0^2 + c h))
     // - tau is monotone in T0 and in c >= 0, so (first t0, cmin) and (last
@@ -403,20 +430,41 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
     // Stage the first K traces, as many as the slabs hold (all when every
     // range fits; block-uniform): the nearest offsets, whose ranges are the
     // shortest.  The other traces' windows are read from global memory as
-    // k_search_fast reads them.  s_pre is nondecreasing, so exactly one
-    // thread sees the boundary s_pre[K] <= capQ < s_pre[K + 1].
-    if (tid == 0) {
-      s_pre[S] = carry;
-      s_wsum[0] = 0;  // every read of the scan scratch is done
-    }
-    __syncthreads();  // s_pre[S], s_wsum[0]
-    K = S;
-    if (__builtin_amdgcn_readfirstlane(carry) > capQ) {  // uniform (carry is)
+    // k_search_fast reads them.
+    if (tid == 0) s_pre[S] = carry;
+    __syncthreads();  // s_pre[S]; every read of the scan scratch is done
+    const int total = __builtin_amdgcn_readfirstlane(carry);  // s_pre[S] (carry is uniform)
+    // The largest k in [0, S] with s_pre[k] <= x (x >= 0, block-uniform):
+    // s_pre is nondecreasing from s_pre[0] = 0, so when s_pre[S] > x exactly
+    // one thread sees the boundary s_pre[k] <= x < s_pre[k + 1].
+    auto largest_le = [&](int x) {
+      if (total <= x) return S;
       for (int k = tid; k < S; k += NT)
-        if (s_pre[k + 1] <= capQ && s_pre[k + 2] > capQ) s_wsum[0] = k + 1;  // k + 1 < S here
+        if (s_pre[k] <= x && s_pre[k + 1] > x) s_wsum[0] = k;
       __syncthreads();
-      K = __builtin_amdgcn_readfirstlane(s_wsum[0]);
+      const int r = __builtin_amdgcn_readfirstlane(s_wsum[0]);
+      __syncthreads();  // read by every thread before the next search writes it
+      return r;
+    };
+    if constexpr (PM == 0) {
+      K = largest_le(cap16);
+    } else if constexpr (PM == 1) {
+      K = Kp = largest_le(cap16 >> 1);
+    } else {
+      if (2 * total <= cap16) {
+        K = Kp = S;
+      } else if (total <= cap16) {
+        K = S;
+        Kp = largest_le(cap16 - total);
+      } else {
+        Kp = largest_le((a.pair_share * cap16) >> 5);
+        K = largest_le(cap16 - __builtin_amdgcn_readfirstlane(s_pre[Kp]));
+      }
     }
+    // The paired slabs take 32 bytes a quad from smp, the sample slabs 16
+    // from sbase, after them.
+    const int pre_kp = PM == 2 ? __builtin_amdgcn_readfirstlane(s_pre[Kp]) : 0;
+    const unsigned sbase = (unsigned)(smp + 32 * pre_kp);
     if (K > 0) {
       // A wave takes G consecutive traces at a time, lanes over their
       // quads, every load of the group in flight before the stores; past the
@@ -428,6 +476,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         __amdgpu_buffer_rsrc_t rs[G];
         int qlo[G], nq[G];
         unsigned dq[G];
+        bool pg[G];   // the trace's slab is paired (uniform)
         float sc[G];  // 16-bit stores: the trace's scale (restored exactly: a power of two)
         int nqm = 0;
 #pragma unroll
@@ -437,8 +486,11 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           qlo[g] = __builtin_amdgcn_readfirstlane(s_lo[k]);
           nq[g] = kg + g < K ? __builtin_amdgcn_readfirstlane(s_pre[k + 1]) - qb : 0;
           nqm = max(nqm, nq[g]);
-          dq[g] = (unsigned)(smp + QB * qb);
-          if (lane == 0 && kg + g < K) s_tr[k].base = (unsigned)(smp + QB * (qb - qlo[g]) - (QB / 4) * a.tau);
+          pg[g] = PM == 1 || (PM == 2 && k < Kp);
+          // paired: 32 bytes a quad from smp; samples: 16 from sbase
+          const int qs = pg[g] ? 32 : 16;
+          dq[g] = pg[g] ? (unsigned)(smp + 32 * qb) : sbase + 16u * (unsigned)(qb - pre_kp);
+          if (lane == 0 && kg + g < K) s_tr[k].base = dq[g] - (unsigned)(qs * qlo[g] + (qs / 4) * a.tau);
           const int row = slc[k].row;
           sc[g] = STORE == SB_STORE_F32 ? 1.f : slc[k].scale;
           rs[g] = __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)a.samples + (int64_t)row * a.stride * ESZ),
@@ -446,11 +498,11 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
         }
         for (int u = lane; u < nqm; u += 64) {
           f4 v[G];
-          float nx[G];  // PR: the sample after the quad (its last difference)
+          float nx[G];  // paired slabs: the sample after the quad (its last difference)
 #pragma unroll
           for (int g = 0; g < G; g++) {
             const unsigned o = (unsigned)(qlo[g] + u) * (4 * ESZ);
-            if constexpr (PR) {
+            if constexpr (PM != 0) {
               if constexpr (STORE == SB_STORE_F32) {
                 nx[g] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs[g], o + 16u, 0, 0));
               } else {
@@ -484,7 +536,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
           for (int g = 0; g < G; g++) {
             if (u >= nq[g]) continue;
             typedef __attribute__((address_space(3))) f4 LF4;
-            if constexpr (PR) {
+            if (pg[g]) {
               const f4 q = v[g];
               const float d0 = __fsub_rn(q.y, q.x), d1 = __fsub_rn(q.z, q.y), d2 = __fsub_rn(q.w, q.z),
                           d3 = __fsub_rn(nx[g], q.w);
@@ -504,14 +556,14 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
       for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
         begin(p, wi.p1);
         if (!wlive) continue;
-        accumulate(S);
+        accumulate(Kp, S);
         finish(p);
       }
     } else {
       for (int p = wi.p0 + wave * CC; p < wi.p1; p += RP) {
         begin(p, wi.p1);
         if (!wlive) continue;
-        accumulate(K);
+        accumulate(Kp, K);
         finish(p);
       }
     }
@@ -538,25 +590,25 @@ __global__ __launch_bounds__(64 * NW) void k_search_lds(SearchArgs a) {
 }
 
 // ------------------------------------------------------------- launchers
-template <int W, int R, int CC, int NW, bool PR>
+template <int W, int R, int CC, int NW, int PM>
 static hipError_t lds_store(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
   const size_t sh = (size_t)a.lds_bytes;
   if (store == SB_STORE_F32)
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F32, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F32, NW, PM>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   else if (store == SB_STORE_F16)
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F16, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_F16, NW, PM>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   else
-    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_BF16, NW, PR>), dim3(nblocks), dim3(64 * NW), sh, st, a);
+    hipLaunchKernelGGL((k_search_lds<W, R, CC, SB_STORE_BF16, NW, PM>), dim3(nblocks), dim3(64 * NW), sh, st, a);
   return hipGetLastError();
 }
 
-template <int R, int CC, int NW, bool PR = false>
+template <int R, int CC, int NW, int PM = 0>
 static hipError_t lds_w(const SearchArgs& a, int store, int nblocks, hipStream_t st) {
   switch (a.w) {
-    case 1: return lds_store<1, R, CC, NW, PR>(a, store, nblocks, st);
-    case 3: return lds_store<3, R, CC, NW, PR>(a, store, nblocks, st);
-    case 5: return lds_store<5, R, CC, NW, PR>(a, store, nblocks, st);
-    case 7: return lds_store<7, R, CC, NW, PR>(a, store, nblocks, st);
+    case 1: return lds_store<1, R, CC, NW, PM>(a, store, nblocks, st);
+    case 3: return lds_store<3, R, CC, NW, PM>(a, store, nblocks, st);
+    case 5: return lds_store<5, R, CC, NW, PM>(a, store, nblocks, st);
+    case 7: return lds_store<7, R, CC, NW, PM>(a, store, nblocks, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -574,8 +626,11 @@ hipError_t launch_search_lds(const SearchArgs& a, int store, const FastVariant& 
   if (nblocks <= 0) return hipSuccess;
   if (a.method != SB_METHOD_CMP || a.lds_bytes <= 0 || a.lds_bytes > kLdsMaxBytes) return hipErrorInvalidValue;
   if (a.lds_bytes < 64 + v.waves * 64 * v.R * 8) return hipErrorInvalidValue;  // the keys at the end
-  if (v.R == 1 && v.CC == 4 && v.waves == 8)
-    return v.pairs ? lds_w<1, 4, 8, true>(a, store, nblocks, st) : lds_w<1, 4, 8>(a, store, nblocks, st);
+  if (v.R == 1 && v.CC == 4 && v.waves == 8) {
+    if (v.pairs == 1) return lds_w<1, 4, 8, 1>(a, store, nblocks, st);
+    if (v.pairs == 2) return lds_w<1, 4, 8, 2>(a, store, nblocks, st);
+    return lds_w<1, 4, 8>(a, store, nblocks, st);
+  }
   if (v.R == 1 && v.CC == 4 && v.waves == 16) return lds_w<1, 4, 16>(a, store, nblocks, st);
   if (v.R == 2 && v.CC == 2 && v.waves == 8) return lds_w<2, 2, 8>(a, store, nblocks, st);
   if (v.R == 2 && v.CC == 2 && v.waves == 16) return lds_w<2, 2, 16>(a, store, nblocks, st);

@@ -730,6 +730,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
     if (v.lds_kb > 0 && a.method == SB_METHOD_CMP && lds_fits(g, a, v, cb, ce)) {
       lds = true;
       a.lds_bytes = v.lds_kb * 1024;
+      a.pair_share = v.share;
     }
     // The global-memory kernel runs kGlobalFallbackVariant when the LDS
     // slabs do not apply (CRS, or CDPs too large to stage).  Both kernels

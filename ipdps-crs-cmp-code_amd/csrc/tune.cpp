@@ -4,10 +4,14 @@
 // of the CDP or supergather).  R and the per-pass parameter count CC are
 // compile-time shapes (kFastVariants); pchunk is a runtime split of the
 // parameter axis across workgroups.  sb_tune times every (variant, pchunk)
-// candidate on the caller's own gather with HIP events and keeps the fastest
-// on the ctx; results are cached per (method, store, w, ncdp, ns, npar).
+// candidate on the caller's own gather with HIP events, then times the three
+// fastest again in interleaved rounds (so that a drift of the clock over the
+// first pass does not decide between candidates a few percent apart) and
+// keeps the one with the lowest median on the ctx; results are cached per
+// (method, store, w, ncdp, ns, npar, ntr).
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <string>
 #include <vector>
@@ -77,8 +81,20 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
   int rc = sb_result_create(ctx, g->ncdp, g->ns, &r);
   if (rc) return rc;
   const int save_v = ctx->fast_variant, save_p = ctx->pchunk_override;
-  double best = DBL_MAX;
-  int best_v = save_v, best_p = save_p;
+  // One timed search of the current shape.
+  auto time_once = [&](double* ms) {
+    int e = method == SB_METHOD_CMP ? sb_cmp_run(ctx, g, (const sb_cmp_params*)params, r)
+                                    : sb_crs_run(ctx, g, (const sb_crs_params*)params, r);
+    sb_timing t;
+    if (!e) e = sb_last_timing(ctx, &t);
+    if (!e) *ms = t.search_ms;
+    return e;
+  };
+  struct Cand {
+    double first;  // summed time of the first pass
+    int v, pc;
+  };
+  std::vector<Cand> pool;
   for (int v = 0; v < kNumFastVariants && !rc; v++) {
     const FastVariant& fv = kFastVariants[v];
     const bool lds = fv.lds_kb > 0;
@@ -91,19 +107,40 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
       if (pc != 0 && pc < 4 * fv.CC) continue;
       ctx->fast_variant = v;
       ctx->pchunk_override = pc;
-      double tsum = 0;
+      double tsum = 0, ms = 0;
       for (int i = 0; i < reps + 1 && !rc; i++) {
-        rc = method == SB_METHOD_CMP ? sb_cmp_run(ctx, g, (const sb_cmp_params*)params, r)
-                                     : sb_crs_run(ctx, g, (const sb_crs_params*)params, r);
-        sb_timing t;
-        if (!rc) rc = sb_last_timing(ctx, &t);
-        if (!rc && i > 0) tsum += t.search_ms;  // first run is a warm-up
+        rc = time_once(&ms);
+        if (!rc && i > 0) tsum += ms;  // first run is a warm-up
       }
       if (rc) break;
-      if (tsum < best) {
-        best = tsum;
-        best_v = v;
-        best_p = pc;
+      pool.push_back(Cand{tsum, v, pc});
+    }
+  }
+  // The three fastest of the first pass, re-timed round by round (one run
+  // of each per round), ranked by their medians.
+  int best_v = save_v, best_p = save_p;
+  if (!rc && !pool.empty()) {
+    std::stable_sort(pool.begin(), pool.end(), [](const Cand& x, const Cand& y) { return x.first < y.first; });
+    const size_t nf = std::min<size_t>(3, pool.size());
+    const int rounds = 2 * reps + 1;
+    std::vector<std::vector<double>> ts(nf);
+    for (int i = 0; i < rounds && !rc; i++)
+      for (size_t f = 0; f < nf && !rc; f++) {
+        ctx->fast_variant = pool[f].v;
+        ctx->pchunk_override = pool[f].pc;
+        double ms = 0;
+        rc = time_once(&ms);
+        ts[f].push_back(ms);
+      }
+    double best = DBL_MAX;
+    for (size_t f = 0; f < nf && !rc; f++) {
+      std::vector<double>& t = ts[f];
+      std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+      const double med = t[t.size() / 2];
+      if (med < best) {
+        best = med;
+        best_v = pool[f].v;
+        best_p = pool[f].pc;
       }
     }
   }

@@ -10,7 +10,11 @@ rounded; the hardware's is within 1 ulp, which moves a range end by at most
 one sample and the counts negligibly), and the LDS layout of an 8-wave
 workgroup: 64 zero bytes, the scan scratch, TraceLds[S], s_pre[S + 1],
 s_lo[S], then the slabs: 16 bytes per quad of samples, or 32 for the shapes
-that stage (sample, difference) pairs.  Test infrastructure only.
+that stage (sample, difference) pairs.  The mixed shapes stage pairs for the
+first Kp traces and samples for traces Kp .. K - 1: every trace as pairs
+when that fits; else, when the samples of every trace fit, every trace with
+as many pairs as the rest of the slab holds; else pairs in at most share/16
+of the slab bytes and samples after them.  Test infrastructure only.
 """
 import numpy as np
 
@@ -37,8 +41,30 @@ def slot_h(cdp_off, geom, dt_us):
     return [np.sort(h[cdp_off[i]:cdp_off[i + 1]], kind="stable") for i in range(len(cdp_off) - 1)]
 
 
-def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w_compiled=None, quad_bytes=16):
-    """Counts of work items by staging case: dict(full, partial, none, dead)."""
+def _largest_le(pre, x):
+    """The largest k with pre[k] <= x (pre nondecreasing from pre[0] = 0 <= x)."""
+    return int(np.searchsorted(pre, x, side="right") - 1)
+
+
+def mixed_split(pre, cap16, share):
+    """(Kp, K) of a mixed-slab workgroup: pre = the exclusive prefix sums of
+    the traces' quads (length S + 1), cap16 = the slab bytes / 16."""
+    S = len(pre) - 1
+    total = int(pre[S])
+    if 2 * total <= cap16:
+        return S, S
+    if total <= cap16:
+        return _largest_le(pre, cap16 - total), S
+    kp = _largest_le(pre, (share * cap16) >> 5)
+    return kp, _largest_le(pre, cap16 - int(pre[kp]))
+
+
+def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w_compiled=None, quad_bytes=16,
+                  share=None):
+    """Counts of work items by staging case: dict(full, partial, none, dead);
+    with share (mixed slabs) also by the traces staged as pairs: pairs_all
+    (Kp = S), mixed_full (0 < Kp < K = S), mixed_partial (0 < Kp <= K < S),
+    pairs_none (Kp = 0)."""
     dt = f32(f32(dt_us) / f32(1e6))
     tau = max(int(f32(tau_s) * (f32(1.0) / dt)), 0)
     W = 2 * tau + 1
@@ -48,9 +74,12 @@ def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w
     hi = ns - tau - 1
     hi2 = f32(hi * hi * (1.0 - 2.0 ** -20)) if hi > 0 else f32(0)
     out = dict(full=0, partial=0, none=0, dead=0)
+    if share is not None:
+        out.update(pairs_all=0, mixed_full=0, mixed_partial=0, pairs_none=0)
     for hs in slot_h(cdp_off, geom, dt_us):
         S = len(hs)
-        capQ = (lds_kb * 1024 - slab_offset(S)) // quad_bytes
+        cap16 = (lds_kb * 1024 - slab_offset(S)) // 16
+        capQ = cap16 // (quad_bytes // 16)
         h2 = hs[S - 2] if S >= 2 else f32(0)
         for t0min in range(0, ns, 64):
             sqlo = f32(f32(t0min) * f32(t0min))
@@ -69,10 +98,14 @@ def staging_cases(cdp_off, geom, dt_us, ns, tau_s, c0, c1, nc, pchunk, lds_kb, w
                 lo = elo >> 2
                 length = np.where(ehi >= elo, ((ehi + W) >> 2) - lo + 1, 0)
                 pre = np.concatenate([[0], np.cumsum(length)])
-                if pre[S] <= capQ:
+                if share is not None:
+                    Kp, K = mixed_split(pre, cap16, share)
+                    out["pairs_all" if Kp == S else "pairs_none" if Kp == 0 else
+                        "mixed_full" if K == S else "mixed_partial"] += 1
+                elif pre[S] <= capQ:
                     K = S
                 else:
-                    K = int(np.searchsorted(pre, capQ, side="right") - 1)  # pre[K] <= capQ < pre[K + 1]
+                    K = _largest_le(pre, capQ)  # pre[K] <= capQ < pre[K + 1]
                 out["full" if K == S else "partial" if K > 0 else "none"] += 1
     return out
 

@@ -7,8 +7,8 @@ parameter grid (count, range, direction), then runs
   - SB_MODE_EXACT: every output bit-identical to the oracle's restatement of
     the reference (CMP float, CRS double, CRS-DE the FP32 twin);
   - SB_MODE_FAST on a drawn launch shape (any global-memory or LDS-staged
-    variant, an automatic or explicit parameter chunk, per-item or persistent
-    grid): the parity bar of tests/parity.py against the reference-precision
+    variant -- sample, paired or mixed slabs --, an automatic or explicit
+    parameter chunk, per-item or persistent grid): the parity bar of tests/parity.py against the reference-precision
     oracle (picks differ only at top-two gaps within 1e-5, or, counted and
     capped, where the FP64 oracle ranks our pick at least as high; coherence
     and stack bit-identical at every agreeing pick and equal to the oracle's
@@ -28,7 +28,7 @@ KEYS_CMP = ("index", "c", "coher", "stack")
 KEYS_CRS = ("index", "a", "b", "c", "coher", "stack")
 
 
-def _cmp_case(sb, i):
+def _cmp_case(sb, i, nvariants):
     r = np.random.default_rng(1000 + i)
     dt_us = int(r.choice([1000, 2000, 4000]))
     fmin = int(r.integers(1, 41))
@@ -48,7 +48,7 @@ def _cmp_case(sb, i):
     line = dict(h_max=float(np.clip(reach * r.uniform(0.5, 1.0), 30.0, 3000.0)),
                 h_jitter=float(r.choice([0.0, 5.0])), nevents=int(r.integers(1, 9)),
                 snr=float(r.choice([2.0, 5.0, 100.0])))
-    variant = int(r.integers(0, 8))
+    variant = int(r.integers(0, nvariants))
     pchunk = int(r.choice([0, 0, 1, 7, 16, 64]))
     persistent = int(variant < 4 and r.random() < 0.3)
     off, geom, s = sb.synth(0, ncdp, fold, ns, dt_us, 2000 + i, **line)
@@ -85,7 +85,7 @@ def _box(r):
 
 @pytest.mark.parametrize("i", range(16))
 def test_cmp_sweep(sb, orc, gpu_ctx, i):
-    desc, (off, geom, s, dt_us), (c0, c1, nc, tau), (variant, pchunk, persistent) = _cmp_case(sb, i)
+    desc, (off, geom, s, dt_us), (c0, c1, nc, tau), (variant, pchunk, persistent) = _cmp_case(sb, i, gpu_ctx.tune_get()[2])
     print(desc, flush=True)
     g = sb.Gather(gpu_ctx, off, geom, s, dt_us)
     try:

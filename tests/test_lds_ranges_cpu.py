@@ -116,3 +116,55 @@ def test_staging_cases_reached_by_the_gpu_tests(sb):
     assert part["full"] > 0 and part["partial"] > 0
     assert paired["full"] > 0 and paired["partial"] > 0
     assert c["none"] > 0
+
+
+def test_mixed_split_fits_and_is_maximal():
+    """The mixed-slab decision (pairs for the first Kp traces, samples for
+    traces Kp .. K - 1) on random quad counts: the slabs never exceed the
+    capacity, Kp <= K, every trace is staged when its samples fit, every
+    trace is paired when its pairs fit, and K is as large as the bytes left
+    after the pairs allow."""
+    import lds_replay as lr
+    rng = np.random.default_rng(7)
+    for _ in range(3000):
+        S = int(rng.integers(1, 60))
+        length = rng.integers(0, 40, S)
+        pre = np.concatenate([[0], np.cumsum(length)])
+        cap16 = int(rng.integers(0, 3 * pre[S] + 2))
+        share = int(rng.choice([4, 8, 16]))
+        Kp, K = lr.mixed_split(pre, cap16, share)
+        assert 0 <= Kp <= K <= S
+        assert 32 * pre[Kp] + 16 * (pre[K] - pre[Kp]) <= 16 * cap16
+        if 2 * pre[S] <= cap16:
+            assert Kp == S
+        elif pre[S] <= cap16:
+            assert K == S and (Kp == S or pre[Kp + 1] + pre[S] > cap16)
+        else:
+            assert 32 * pre[Kp] <= (share * cap16 >> 5) * 32
+            assert K == S or pre[K + 1] + pre[Kp] > cap16
+
+
+def test_mixed_slab_cases_reached_by_the_gpu_tests(sb):
+    """For the mixed-slab shapes (52 KiB, pairs in at most 16/16 or 8/16 of
+    the slab bytes), the gathers of tests/test_gpu_lds.py run workgroups
+    that pair every trace, that stage every trace with pairs for the first
+    ones, that stage pairs, samples and leave the rest in global memory, and
+    that stage samples only."""
+    import lds_replay as lr
+    for share in (16, 8):
+        seen = {}
+        # test_lds_partial_staging: 128 and 256 parameters per workgroup
+        off, geom, _ = sb.synth(0, 5, (60, 120), 1501, 2000, 31, h_max=1500.0, h_jitter=5.0, nevents=8)
+        for pc in (128, 256):
+            c = lr.staging_cases(off, geom, 2000, 1501, 0.002, 1.98e-7, 1.77e-6, 256, pc, 52, share=share)
+            for k, v in c.items():
+                seen[k] = seen.get(k, 0) + v
+        # test_lds_ranges_beyond_lds: the automatic split (32 parameters on 256 CUs) and 200
+        off, geom, _ = sb.synth(0, 3, (90, 120), 2001, 2000, 5, h_max=2500.0, nevents=5)
+        for pc in (32, 200):
+            c = lr.staging_cases(off, geom, 2000, 2001, 0.006, 1.0e-7, 1.5e-6, 200, pc, 52, share=share)
+            for k, v in c.items():
+                seen[k] = seen.get(k, 0) + v
+        print("mixed slabs, share %d/16: %s" % (share, seen))
+        for k in ("pairs_all", "mixed_full", "mixed_partial", "pairs_none"):
+            assert seen[k] > 0, (share, k)
