@@ -204,7 +204,7 @@ def cpu_baseline(sb, cdp_off, geom, samples, every=4):
         return dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port", sample=sample)
 
 
-def cli_config0(sb):
+def cli_cmp_200cdp(sb):
     """The reference's own CPU-runnable case end to end through the drop-in
     CLI: a 200-CDP line, fold 60 at half-offsets 0-1475 m (25 m steps), ns
     1001, dt 2 ms, nc 101, tau 2 ms, written as an SU file.  Runs the `cmp`
@@ -377,8 +377,19 @@ def side_configs(sb, args):
             tuples = tk[0]["tuples"]
             out["crs_de"][store] = dict(value=tuples * 2 / wall, ms_per_step=wall / 2 * 1e3, tuples=tuples,
                                         search_ms=ms, roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms * 1e-3) /
-                                        1e12 / PEAK_FP32_VECTOR_TFLOPS)
+                                        1e12 / PEAK_FP32_VECTOR_TFLOPS, restored=tk[0]["restored"])
             print("bench: crs-de %s %.4g evals/s (%.1f ms)" % (store, out["crs_de"][store]["value"], ms), flush=True)
+            if store != "f32":
+                ctx.set_option(sb.OPT_RESTORE_FP32, 0)
+                try:
+                    wall2, tk2 = timed_steps(sb, ctx, g, prm, 2, 1)
+                finally:
+                    ctx.set_option(sb.OPT_RESTORE_FP32, 1)
+                ms2 = float(np.mean([t["search_ms"] for t in tk2]))
+                out["crs_de"][store + "_direct"] = dict(value=tuples * 2 / wall2, ms_per_step=wall2 / 2 * 1e3,
+                                                        search_ms=ms2)
+                print("bench: crs-de %s without the FP32 copy %.4g evals/s (%.1f ms)" %
+                      (store, out["crs_de"][store + "_direct"]["value"], ms2), flush=True)
             g.close()
     if not args.no_cpu:
         cpu = de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, [100, 101, 102, 103])
@@ -597,9 +608,28 @@ def main():
             fin_ms = float(np.mean([t["finalize_ms"] for t in tk]))
             results[store] = dict(value=tuples * args.steps / wall, ms_per_step=wall / args.steps * 1e3,
                                   search_ms=search_ms, finalize_ms=fin_ms, tuples=tuples, tune=(tk[0]["variant"], ctx.tune_get()[1]),
-                                  blocks=tk[0]["grid_blocks"], lds_staged=tk[0]["lds_staged"])
+                                  blocks=tk[0]["grid_blocks"], lds_staged=tk[0]["lds_staged"],
+                                  restored=tk[0]["restored"])
             print("bench: %s %.4g evals/s, search %.3f ms, finalize %.3f ms" %
                   (store, results[store]["value"], search_ms, fin_ms), flush=True)
+            if store != "f32":
+                # the same step with the 16-bit kernels reading the store
+                # (no transient FP32 copy), tuned for that
+                retune = args.variant < 0 and not args.no_tune
+                ctx.set_option(sb.OPT_RESTORE_FP32, 0)
+                try:
+                    if retune:
+                        sb.tune(ctx, g, prm, reps=4)
+                    wall2, tk2 = timed_steps(sb, ctx, g, prm, args.steps, args.warmup)
+                    direct_launch = {"variant": tk2[0]["variant"], "pchunk": ctx.tune_get()[1]}
+                finally:
+                    ctx.set_option(sb.OPT_RESTORE_FP32, 1)
+                    if retune:
+                        sb.tune(ctx, g, prm, reps=4)  # the shape tuned with the copy (cached)
+                results[store]["direct"] = dict(value=tuples * args.steps / wall2,
+                                                ms_per_step=wall2 / args.steps * 1e3, launch=direct_launch)
+                print("bench: %s without the FP32 copy %.4g evals/s" % (store, results[store]["direct"]["value"]),
+                      flush=True)
             if not args.no_modes:
                 results[store]["modes"] = launch_modes(sb, ctx, g, prm)
                 print("bench: %s launch modes %s" % (store, results[store]["modes"]), flush=True)
@@ -644,7 +674,10 @@ def main():
                               "lds_staged": f32["lds_staged"]}},
         "fp16_store": None if f16 is None else {
             "value": f16["value"], "ms_per_step": f16["ms_per_step"], "speedup_vs_f32": f16["value"] / f32["value"],
-            "launch": {"variant": f16["tune"][0], "pchunk": f16["tune"][1]}},
+            "launch": {"variant": f16["tune"][0], "pchunk": f16["tune"][1]}, "restored": f16["restored"],
+            "direct": f16.get("direct"),
+            "what": "restored: the search read a transient FP32 copy of the rows (SB_OPT_RESTORE_FP32, the "
+                    "default; its time is inside the step); direct: the 16-bit kernels reading the store"},
         "launch_modes_ms": {k: results[k].get("modes") for k in results},
         "panel_latency_ms": {"p50": p50, "p99": p99, "what": "1-CDP CMP panel, upload+search+download, sequential"},
         "value_evaluated": f32["value"] * (1.0 - skip),
@@ -682,8 +715,8 @@ def main():
     if not args.no_side:
         del cdp_off, geom, samples
         line.update(side_configs(sb, args))
-        print("bench: config-0 line through the CLI", flush=True)
-        line["cli_config0"] = cli_config0(sb)
+        print("bench: CMP 200-CDP line through the cmp program", flush=True)
+        line["cli_cmp_200cdp"] = cli_cmp_200cdp(sb)
     if not args.no_serving:
         lines = serving_lines(sb)
         line["serving"] = serving_bench(sb, lines)

@@ -278,6 +278,8 @@ typedef struct {
   int variant;         /* launch shape that ran (sb_tune_get numbering; -1: exact mode) */
   int lds_staged;      /* 1 if the LDS-staged kernel ran (windows from LDS slabs; traces
                           whose ranges did not fit read from global memory) */
+  int restored;        /* 1 if the search read a transient FP32 copy of a 16-bit store's
+                          rows (SB_OPT_RESTORE_FP32; search_ms includes the copy) */
 } sb_timing;
 int sb_last_timing(const sb_ctx* ctx, sb_timing* t);
 
@@ -308,10 +310,21 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
  * A term tells dips on either side apart.  An extension behind crs-ocl-de's
  * -azimuth flag, which the reference documents but does not ship
  * (README.md:94,115,131). */
+/* SB_OPT_RESTORE_FP32 = 1 (default): an SB_MODE_FAST search of a 16-bit
+ * store (CMP, CRS grid, CRS-DE) first restores the rows it reads to FP32 --
+ * the stored value times the trace's scale, the values the 16-bit kernels
+ * restore after every load -- in a transient copy on the ctx (counted against
+ * its budget, released when the search's work is done), then runs the FP32
+ * kernels on it.  0, or a copy that does not fit the budget, or a search
+ * being captured into a graph: the 16-bit kernels read the store directly,
+ * restoring every window after its load.  Both evaluate the same FP32
+ * values with the same FP32 arithmetic: the results are the same either
+ * way.  No reference counterpart (the reference has no 16-bit store). */
 enum {
   SB_OPT_PERSISTENT = 1,
   SB_OPT_STREAM_ROWS = 2, /* traces per sb_search_streamed batch, 0 = auto */
-  SB_OPT_DM_AZIMUTH = 3
+  SB_OPT_DM_AZIMUTH = 3,
+  SB_OPT_RESTORE_FP32 = 4
 };
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);

@@ -18,6 +18,7 @@ MODE_FAST, MODE_EXACT = 0, 1
 OPT_PERSISTENT = 1
 OPT_STREAM_ROWS = 2
 OPT_DM_AZIMUTH = 3
+OPT_RESTORE_FP32 = 4
 GRAPH_CRS_DE = 2
 ERR_DEADLINE = 9
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
@@ -65,7 +66,7 @@ class DeviceArrays(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("search_ms", C.c_double), ("finalize_ms", C.c_double), ("total_ms", C.c_double),
                 ("tuples", C.c_double), ("grid_blocks", C.c_int), ("block_threads", C.c_int),
-                ("variant", C.c_int), ("lds_staged", C.c_int)]
+                ("variant", C.c_int), ("lds_staged", C.c_int), ("restored", C.c_int)]
 
 
 class SynthSpec(C.Structure):

@@ -140,12 +140,13 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
   // Software-pipelined by one trace: the window of trace k+1 is loaded
   // while trace k accumulates (one load per candidate and trace, so without
   // this every wave waits out each load's latency).  Same order of sums.
-  // FP16 windows stay halves and feed mixed-precision FMAs (v_fma_mix_f32);
-  // BF16 and FP32 windows are FP32.
+  // 16-bit windows are restored to FP32 (stored value times the trace's
+  // power-of-two scale, exact) and evaluated like FP32 ones, so a 16-bit
+  // store gives the FP32 store's results on the restored samples -- which is
+  // what the search on a restored FP32 copy (SB_OPT_RESTORE_FP32) computes.
   const float T0sq = __fmul_rn(T0, T0);
   const float K1 = __fmul_rn(__fmul_rn(2.f, T0), pa), K2 = fmaf(pa, pa, pb);
-  constexpr bool HALF = STORE == SB_STORE_F16;
-  typedef typename std::conditional<HALF, WinH<W + 1>, Win<W + 1>>::type WinT;
+  typedef Win<W + 1> WinT;
   auto issue = [&](const SlotF& s, WinT& sv, float& x, bool& ok) {
     // the slot carries its row's address: no per-trace address arithmetic
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)s.addr, (short)0, nrec, kRsrcWord3);
@@ -159,40 +160,18 @@ __device__ void de_eval_fast(const SearchArgs& a, int k0, int k1, float T0, floa
     if constexpr (W == 1) ok = ok && tt >= 0.f;
     x = __builtin_amdgcn_fractf(ts);
     const unsigned off = ok ? (STORE == SB_STORE_F32 ? (unsigned)e * 4u : ((unsigned)e & ~1u) * 2u) : kOob;
-    if constexpr (HALF)
-      sv = load_win_h<W + 1>(rs, off, ((unsigned)e & 1u) << 4);
-    else
-      sv = load_win<W + 1, STORE>(rs, off, ((unsigned)e & 1u) << 4);
+    sv = load_win<W + 1, STORE>(rs, off, ((unsigned)e & 1u) << 4);
   };
   auto consume = [&](const SlotF& s, const WinT& sv, float x, bool ok) {
     cnt += ok ? 1 : 0;
-    if constexpr (STORE == SB_STORE_F32) {
+    float r[W + 1];
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        const float v = fmaf(sv.v[j + 1] - sv.v[j], x, sv.v[j]);
-        num[j] += v;
-        den = fmaf(v, v, den);
-      }
-    } else if constexpr (HALF) {
-      // v = s_j + x s_{j+1} - x s_j in two mixed-precision FMAs
-      float q = 0.f;
+    for (int j = 0; j <= W; j++) r[j] = STORE == SB_STORE_F32 ? sv.v[j] : sv.v[j] * s.scale;
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        const float hj = (float)sv.h[j], hj1 = (float)sv.h[j + 1];
-        const float v = fmaf(-x, hj, fmaf(x, hj1, hj));
-        num[j] = fmaf(s.scale, v, num[j]);
-        q = fmaf(v, v, q);
-      }
-      den = fmaf(s.scale2, q, den);
-    } else {
-      float q = 0.f;
-#pragma unroll
-      for (int j = 0; j < W; j++) {
-        const float v = fmaf(sv.v[j + 1] - sv.v[j], x, sv.v[j]);
-        num[j] = fmaf(s.scale, v, num[j]);
-        q = fmaf(v, v, q);
-      }
-      den = fmaf(s.scale2, q, den);
+    for (int j = 0; j < W; j++) {
+      const float v = fmaf(r[j + 1] - r[j], x, r[j]);
+      num[j] += v;
+      den = fmaf(v, v, den);
     }
   };
   // Slots are read two traces ahead of their use, so the scalar loads are in

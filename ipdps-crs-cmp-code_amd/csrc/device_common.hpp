@@ -48,8 +48,8 @@ struct Win {
   float v[N];
 };
 
-// N consecutive FP16 samples kept as halves (no conversion): the
-// interpolation consumes them directly in mixed-precision FMAs.
+// N consecutive 16-bit samples as loaded (no conversion), before load_win
+// restores them to FP32.
 template <int N>
 struct WinH {
   _Float16 h[N];

@@ -221,6 +221,47 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ src,
   }
 }
 
+// Rows of a 16-bit store restored to FP32 -- the stored value times the
+// row's power-of-two scale, the product every 16-bit search path forms --
+// padding included, into a transient FP32 copy with the same row stride.
+// Eight samples per thread (one 16-byte load, two 16-byte stores); rows of
+// stride elements, stride a multiple of 64.
+template <int STORE>
+__global__ __launch_bounds__(256) void k_restore_rows(const uint4* __restrict__ src, const float* __restrict__ scale,
+                                                      int64_t stride, int64_t nvec, float4* __restrict__ dst) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvec) return;
+  const float sc = scale[(v * 8) / stride];
+  const uint4 q = src[v];
+  float f[8];
+  const unsigned w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if constexpr (STORE == SB_STORE_F16) {
+      typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+      const h2v h = __builtin_bit_cast(h2v, w[k]);
+      f[2 * k] = (float)h.x * sc;
+      f[2 * k + 1] = (float)h.y * sc;
+    } else {
+      f[2 * k] = __uint_as_float(w[k] << 16) * sc;
+      f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u) * sc;
+    }
+  }
+  dst[2 * v] = make_float4(f[0], f[1], f[2], f[3]);
+  dst[2 * v + 1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+// A CRS slot table whose row addresses point into a restored FP32 copy:
+// addr = base + row * row_bytes (base: the copy's row-0 address).
+__global__ __launch_bounds__(256) void k_rebase_slots(const SlotF* __restrict__ src, int64_t n, uint64_t base,
+                                                      uint64_t row_bytes, SlotF* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  SlotF s = src[i];
+  s.addr = base + (uint64_t)s.row * row_bytes;
+  dst[i] = s;
+}
+
 // Plain FP32 copy into the padded row layout.
 __global__ __launch_bounds__(256) void k_pad_rows(const float* __restrict__ src, int ns, int64_t stride,
                                                   float* __restrict__ dst) {
@@ -1164,6 +1205,28 @@ hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int s
     hipLaunchKernelGGL(k_quantize<SB_STORE_F16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
   else
     hipLaunchKernelGGL(k_quantize<SB_STORE_BF16>, dim3(ntr), dim3(256), 0, st, src, ns, stride, dst, scale);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_restore_rows(const void* src, const float* scale, int64_t nrows, int64_t stride, int store,
+                               float* dst, hipStream_t st) {
+  if (nrows <= 0) return hipSuccess;
+  if (store == SB_STORE_F32 || stride % 64 != 0) return hipErrorInvalidValue;
+  const int64_t nvec = nrows * stride / 8;
+  const dim3 grid((unsigned)((nvec + 255) / 256));
+  if (store == SB_STORE_F16)
+    hipLaunchKernelGGL(k_restore_rows<SB_STORE_F16>, grid, dim3(256), 0, st, (const uint4*)src, scale, stride, nvec,
+                       (float4*)dst);
+  else
+    hipLaunchKernelGGL(k_restore_rows<SB_STORE_BF16>, grid, dim3(256), 0, st, (const uint4*)src, scale, stride, nvec,
+                       (float4*)dst);
+  return SB_CHECK_LAUNCH();
+}
+
+hipError_t launch_rebase_slots(const SlotF* src, int64_t n, uint64_t base, uint64_t row_bytes, SlotF* dst,
+                               hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rebase_slots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, n, base, row_bytes, dst);
   return SB_CHECK_LAUNCH();
 }
 

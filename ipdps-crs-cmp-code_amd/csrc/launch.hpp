@@ -78,6 +78,14 @@ hipError_t launch_crs_slots(int cdp0, int ncdp, const int32_t* cdp_off, const in
 hipError_t launch_store(const float* src, int ntr, int ns, int64_t stride, int store, void* dst, float* scale,
                         hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
+// nrows rows of a 16-bit store (src: the first row, scale: its scale; rows
+// of stride elements, a multiple of 64) restored to FP32 into dst, padding
+// included: the values every 16-bit search path restores.
+hipError_t launch_restore_rows(const void* src, const float* scale, int64_t nrows, int64_t stride, int store,
+                               float* dst, hipStream_t st);
+// n CRS fast slots copied with addr = base + row * row_bytes.
+hipError_t launch_rebase_slots(const SlotF* src, int64_t n, uint64_t base, uint64_t row_bytes, SlotF* dst,
+                               hipStream_t st);
 
 // a.queue set: persistent grid (at most what the CUs hold at once, a
 // multiple of 8) pulling the nblocks work items; else one block per item.

@@ -66,7 +66,11 @@ static void release_cache(sb_ctx* ctx) {
   ctx->cache.clear();
 }
 
-int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
+// transient: the block is used only by work queued on the ctx stream, which
+// is ordered after the free of a cached block (no host wait), and a
+// shortage returns SB_ERR_BUDGET without an error message (the caller has
+// another way).
+static int alloc_block(sb_ctx* ctx, size_t bytes, void** p, bool transient) {
   *p = nullptr;
   if (bytes == 0) return SB_OK;
   const size_t want = round_block(bytes);
@@ -76,8 +80,10 @@ int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
     // The new owner may write the block from any stream (or the host): wait
     // until the work queued on the ctx stream before the free has finished.
     if (it->second.freed) {
-      const hipError_t e = hipEventSynchronize(it->second.freed);
-      if (e != hipSuccess) return hip_fail(e, "cached block");
+      if (!transient) {
+        const hipError_t e = hipEventSynchronize(it->second.freed);
+        if (e != hipSuccess) return hip_fail(e, "cached block");
+      }
       drop_event(ctx, it->second.freed);
     }
     *p = it->second.p;
@@ -86,22 +92,33 @@ int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) {
     return SB_OK;
   }
   if (ctx->used + want > ctx->budget) release_cache(ctx);
-  if (ctx->used + want > ctx->budget)
+  if (ctx->used + want > ctx->budget) {
+    if (transient) return SB_ERR_BUDGET;
     return fail(SB_ERR_BUDGET, "HBM budget exceeded: need " + std::to_string(want) + " B, " +
                                    std::to_string(ctx->budget - ctx->used) + " B left");
+  }
   hipError_t e = hipMalloc(p, want);
   if (e != hipSuccess && !ctx->cache.empty()) {
+    (void)hipGetLastError();
     release_cache(ctx);
     e = hipMalloc(p, want);
   }
   if (e != hipSuccess) {
     *p = nullptr;
+    if (transient) {
+      (void)hipGetLastError();  // leave no error for the next launch check
+      return SB_ERR_BUDGET;
+    }
     return hip_fail(e, "hipMalloc");
   }
   ctx->used += want;
   ctx->sizes[*p] = want;
   return SB_OK;
 }
+
+int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p) { return alloc_block(ctx, bytes, p, false); }
+
+int ctx_alloc_transient(sb_ctx* ctx, size_t bytes, void** p) { return alloc_block(ctx, bytes, p, true); }
 
 // Returns a block to the cache, stream-ordered: an event recorded on the ctx
 // stream marks the point after which no queued work of this ctx reads it.
@@ -368,14 +385,21 @@ int GatherBuild::begin(sb_ctx* c, int ncdp, const int32_t* cdp_off, int ns, int 
     G->crs_slot_off.resize(ncdp + 1);
     G->crs_slot_off[0] = 0;
     nbr_reach.resize(ncdp);
+    G->crs_row_lo.resize(ncdp);
+    G->crs_row_hi.resize(ncdp);
     int32_t reach = 0;
     for (int i = 0; i < ncdp; i++) {
       int64_t sz = 0;
+      int32_t lo = ncdp, hi = -1;
       for (int q = nbr_off[i]; q < nbr_off[i + 1]; q++) {
         sz += cdp_off[nbr[q] + 1] - cdp_off[nbr[q]];
         reach = std::max(reach, nbr[q]);
+        lo = std::min(lo, nbr[q]);
+        hi = std::max(hi, nbr[q]);
       }
       nbr_reach[i] = reach;
+      G->crs_row_lo[i] = cdp_off[lo];
+      G->crs_row_hi[i] = cdp_off[hi + 1];
       if (G->crs_slot_off[i] + sz > INT32_MAX) return fail(SB_ERR_INVALID, "supergathers too large");
       G->crs_slot_off[i + 1] = G->crs_slot_off[i] + (int32_t)sz;
     }
@@ -644,6 +668,95 @@ int check_range(const sb_ctx* ctx, const sb_gather* g, const sb_result* r, int c
 // Timing events are not recorded while a graph is being captured.
 hipError_t record(sb_ctx* ctx, int i) { return ctx->capturing ? hipSuccess : hipEventRecord(ctx->ev[i], ctx->stream); }
 
+// SB_OPT_RESTORE_FP32: the rows a fast search of a 16-bit store reads,
+// restored to FP32 in a transient copy (same row stride, kSamplesLead bytes
+// in front as in an FP32 store), so that the FP32 kernels run on the values
+// the 16-bit kernels restore after each load; CRS-DE also gets a copy of its
+// slot table with row addresses in the copy.  The blocks go back to the ctx
+// cache (stream-ordered) when this goes out of scope -- or, in the eager run
+// of sb_graph_create, to the graph, whose capture then restores into them.
+struct Restored {
+  sb_ctx* ctx = nullptr;
+  bool owned = false;  // the blocks are this search's (else the graph's)
+  void* mem = nullptr;
+  void* slots = nullptr;
+  int64_t r0 = 0, r1 = 0;  // rows of the copy
+  int so0 = 0, so1 = 0;    // rebased slots
+  ~Restored() {
+    if (!owned) return;
+    if (ctx->restore_keep) {  // handed to the graph being created
+      ctx->kept_mem = mem;
+      ctx->kept_slots = slots;
+      ctx->kept_mem_bytes = mem_bytes();
+      ctx->kept_slots_bytes = slots_bytes();
+      return;
+    }
+    ctx_free(ctx, mem);
+    ctx_free(ctx, slots);
+  }
+  size_t mem_bytes() const { return (size_t)kSamplesLead + (size_t)(r1 - r0) * stride * 4; }
+  size_t slots_bytes() const { return (size_t)(so1 - so0) * sizeof(SlotF); }
+  int64_t stride = 0;
+  // Sets up the copy for CDPs [cb, ce) (CMP: their rows; CRS: the rows of
+  // their supergathers); false, with nothing held, when it does not apply
+  // (FP32 store, option off, a graph capture without blocks kept for it) or
+  // does not fit the budget.
+  bool prepare(sb_ctx* c, const sb_gather* g, int cb, int ce, bool crs, bool rebase) {
+    if (g->store == SB_STORE_F32 || !c->restore_fp32) return false;
+    stride = g->stride;
+    if (crs) {
+      r0 = g->crs_row_lo[cb];
+      r1 = g->crs_row_hi[cb];
+      for (int i = cb + 1; i < ce; i++) {
+        r0 = std::min<int64_t>(r0, g->crs_row_lo[i]);
+        r1 = std::max<int64_t>(r1, g->crs_row_hi[i]);
+      }
+    } else {
+      r0 = g->cdp_off[cb];
+      r1 = g->cdp_off[ce];
+    }
+    if (rebase) {
+      so0 = g->crs_slot_off[cb];
+      so1 = g->crs_slot_off[ce];
+    }
+    if (c->capturing) {
+      // no allocation under capture: the blocks the eager run kept
+      if (!c->graph_rest_mem || c->graph_rest_mem_bytes < mem_bytes()) return false;
+      if (rebase && (!c->graph_rest_slots || c->graph_rest_slots_bytes < slots_bytes())) return false;
+      ctx = c;
+      mem = c->graph_rest_mem;
+      slots = rebase ? c->graph_rest_slots : nullptr;
+      return true;
+    }
+    if (ctx_alloc_transient(c, mem_bytes(), &mem) != SB_OK) return false;
+    if (rebase && ctx_alloc_transient(c, slots_bytes(), &slots) != SB_OK) {
+      ctx_free(c, mem);
+      mem = nullptr;
+      return false;
+    }
+    ctx = c;
+    owned = true;
+    return true;
+  }
+  // Row 0 of the copy as the kernels address it (rows below r0 are never read).
+  uintptr_t row0(const sb_gather* g) const {
+    return (uintptr_t)mem + (uintptr_t)kSamplesLead - (uintptr_t)(r0 * g->stride * 4);
+  }
+  // The search's arguments on the copy (FP32 store from here on).
+  void apply(const sb_gather* g, SearchArgs& a) const {
+    a.samples = (const void*)row0(g);
+    if (slots) a.slots = (const SlotF*)((uintptr_t)slots - (uintptr_t)so0 * sizeof(SlotF));
+  }
+  hipError_t launch(const sb_gather* g) const {
+    hipError_t e = launch_restore_rows((const char*)g->d_samples + r0 * g->stride * 2, g->d_scale + r0, r1 - r0,
+                                       g->stride, g->store, (float*)((char*)mem + kSamplesLead), ctx->stream);
+    if (e == hipSuccess && slots)
+      e = launch_rebase_slots(g->d_crs_slots + so0, so1 - so0, (uint64_t)row0(g), (uint64_t)g->stride * 4,
+                              (SlotF*)slots, ctx->stream);
+    return e;
+  }
+};
+
 // The work table a launch reads: a graph being captured gets its own copy.
 const WorkItem* work_table(const sb_ctx* ctx) {
   return (const WorkItem*)(ctx->graph_work ? ctx->graph_work : ctx->work_buf);
@@ -719,13 +832,20 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   // Reference arithmetic on the tiled layout (keys + winner re-evaluation)
   // where it is compiled; otherwise one thread per (CDP, t0).
   const bool tiled_exact = mode == SB_MODE_EXACT && exact_tiled_supported(a.method, store, a.w);
+  // A 16-bit store's fast search on a restored FP32 copy (SB_OPT_RESTORE_FP32;
+  // not for the allocation-free entry point): kstore is the store the search
+  // kernel reads.  The winners are re-evaluated from the store itself.
+  Restored rest;
+  const bool restored =
+      mode == SB_MODE_FAST && !sc.base && rest.prepare(ctx, g, cb, ce, a.method == SB_METHOD_CRS, false);
+  const int kstore = restored ? SB_STORE_F32 : store;
   // The work table's tiling: family (for the cache key), t0 tiles and
   // parameters per workgroup.
   int family, ntiles, npar_w, pchunk;
   if (mode == SB_MODE_FAST) {
     variant = ctx->fast_variant >= 0 ? ctx->fast_variant
-              : store == SB_STORE_F32 ? kDefaultFastVariantF32
-                                      : kDefaultFastVariant;
+              : kstore == SB_STORE_F32 ? kDefaultFastVariantF32
+                                       : kDefaultFastVariant;
     v = kFastVariants[variant];
     if (v.lds_kb > 0 && a.method == SB_METHOD_CMP && lds_fits(g, a, v, cb, ce)) {
       lds = true;
@@ -792,10 +912,16 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
     if (e != hipSuccess) return hip_fail(e, "queue clear");
   }
   HIPR(record(ctx, 1));
+  SearchArgs as = a;  // the search kernel's view of the samples
+  if (restored) {
+    rest.apply(g, as);
+    e = rest.launch(g);
+    if (e != hipSuccess) return hip_fail(e, "FP32 restore");
+  }
   if (mode == SB_MODE_FAST && lds)
-    e = launch_search_lds(a, store, v, nblocks, ctx->stream);
+    e = launch_search_lds(as, kstore, v, nblocks, ctx->stream);
   else if (mode == SB_MODE_FAST)
-    e = launch_search_fast(a, store, variant, nblocks, ctx->num_cus, ctx->stream);
+    e = launch_search_fast(as, kstore, variant, nblocks, ctx->num_cus, ctx->stream);
   else if (tiled_exact)
     e = launch_search_exact_tiled(a, nblocks, ctx->stream);
   else
@@ -813,6 +939,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   ctx->last.block_threads = mode == SB_MODE_FAST || tiled_exact ? (lds ? 64 * v.waves : 256) : 64;
   ctx->last.variant = variant;
   ctx->last.lds_staged = lds ? 1 : 0;
+  ctx->last.restored = restored ? 1 : 0;
   ctx->timing_pending = true;
   ctx->timing_graph = false;
   return SB_OK;
@@ -1041,9 +1168,20 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
     return rc;
   a.work = work_table(ctx);
   a.nwork = nblocks;
+  // A 16-bit store's fast search on a restored FP32 copy of its supergather
+  // rows, with a slot table addressing the copy (SB_OPT_RESTORE_FP32).
+  Restored rest;
+  const bool restored = de_mode == SB_MODE_FAST && rest.prepare(ctx, g, cb, ce, true, true);
   HIPR(record(ctx, 0));
   HIPR(record(ctx, 1));
-  hipError_t e = launch_de(a, d, g->store, de_mode == SB_MODE_EXACT, nblocks, ctx->stream);
+  SearchArgs as = a;
+  hipError_t e = hipSuccess;
+  if (restored) {
+    rest.apply(g, as);
+    e = rest.launch(g);
+    if (e != hipSuccess) return hip_fail(e, "FP32 restore");
+  }
+  e = launch_de(as, d, restored ? SB_STORE_F32 : g->store, de_mode == SB_MODE_EXACT, nblocks, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "CRS-DE launch");
   HIPR(record(ctx, 2));
   HIPR(record(ctx, 3));
@@ -1052,6 +1190,7 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
   ctx->last.grid_blocks = nblocks;
   ctx->last.block_threads = 256;
   ctx->last.variant = -1;
+  ctx->last.restored = restored ? 1 : 0;
   ctx->timing_pending = true;
   ctx->timing_graph = false;
   return SB_OK;
@@ -1122,6 +1261,10 @@ extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
     ctx->dm_azimuth_mdeg = value;
     return SB_OK;
   }
+  if (option == SB_OPT_RESTORE_FP32) {
+    ctx->restore_fp32 = value ? 1 : 0;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -1139,6 +1282,10 @@ extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
     *value = ctx->dm_azimuth_mdeg;
     return SB_OK;
   }
+  if (option == SB_OPT_RESTORE_FP32) {
+    *value = ctx->restore_fp32;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -1148,6 +1295,11 @@ struct sb_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void* work = nullptr;  // the graph's copy of the work table
+  // a 16-bit store's restored FP32 copy (and rebased CRS-DE slots), which
+  // every replay rewrites (SB_OPT_RESTORE_FP32)
+  void* rest_mem = nullptr;
+  void* rest_slots = nullptr;
+  size_t rest_mem_bytes = 0, rest_slots_bytes = 0;
 };
 
 extern "C" void sb_graph_destroy(sb_graph* gr) {
@@ -1157,6 +1309,8 @@ extern "C" void sb_graph_destroy(sb_graph* gr) {
   if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
   if (gr->graph) (void)hipGraphDestroy(gr->graph);
   ctx_free(gr->ctx, gr->work);
+  ctx_free(gr->ctx, gr->rest_mem);
+  ctx_free(gr->ctx, gr->rest_slots);
   delete gr;
 }
 
@@ -1170,19 +1324,32 @@ extern "C" int sb_graph_create(sb_ctx* ctx, const sb_gather* g, int kind, const 
     if (kind == SB_GRAPH_CRS_DE) return sb_crs_de_run_range(ctx, g, (const sb_de_params*)params, cb, ce, r);
     return fail(SB_ERR_INVALID, "sb_graph_create: bad kind");
   };
-  // 1. eager run: validates the arguments and builds the work table.
+  // 1. eager run: validates the arguments and builds the work table (and
+  // keeps a 16-bit store's restored copy for the graph).
+  ctx->restore_keep = true;
+  ctx->kept_mem = ctx->kept_slots = nullptr;
   int rc = run();
+  ctx->restore_keep = false;
+  std::unique_ptr<sb_graph, void (*)(sb_graph*)> G(new sb_graph, sb_graph_destroy);
+  G->ctx = ctx;
+  G->rest_mem = ctx->kept_mem;
+  G->rest_slots = ctx->kept_slots;
+  G->rest_mem_bytes = ctx->kept_mem_bytes;
+  G->rest_slots_bytes = ctx->kept_slots_bytes;
+  ctx->kept_mem = ctx->kept_slots = nullptr;
   if (rc) return rc;
   HIPR(hipSetDevice(ctx->device));
   HIPR(hipStreamSynchronize(ctx->stream));
-  std::unique_ptr<sb_graph, void (*)(sb_graph*)> G(new sb_graph, sb_graph_destroy);
-  G->ctx = ctx;
   const size_t bytes = std::max<size_t>((size_t)ctx->work_n, 1) * sizeof(WorkItem);
   if ((rc = ctx_alloc(ctx, bytes, &G->work))) return rc;
   HIPR(hipMemcpy(G->work, ctx->work_buf, bytes, hipMemcpyDeviceToDevice));
   // 2. capture the same calls: the table is cached (same key), so nothing
   // but kernel launches reaches the stream.
   ctx->graph_work = G->work;
+  ctx->graph_rest_mem = G->rest_mem;
+  ctx->graph_rest_slots = G->rest_slots;
+  ctx->graph_rest_mem_bytes = G->rest_mem_bytes;
+  ctx->graph_rest_slots_bytes = G->rest_slots_bytes;
   ctx->capturing = true;
   hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
@@ -1196,6 +1363,8 @@ extern "C" int sb_graph_create(sb_ctx* ctx, const sb_gather* g, int kind, const 
   }
   ctx->capturing = false;
   ctx->graph_work = nullptr;
+  ctx->graph_rest_mem = ctx->graph_rest_slots = nullptr;
+  ctx->graph_rest_mem_bytes = ctx->graph_rest_slots_bytes = 0;
   ctx->timing_pending = false;
   if (rc) return rc;
   HIPR(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));

@@ -49,9 +49,20 @@ struct sb_ctx {
   int persistent = 0;          // SB_OPT_PERSISTENT
   int stream_rows = 0;         // SB_OPT_STREAM_ROWS (0: automatic)
   int dm_azimuth_mdeg = -1;    // SB_OPT_DM_AZIMUTH (-1: the reference's unsigned distance)
+  int restore_fp32 = 1;        // SB_OPT_RESTORE_FP32
   void* scratch = nullptr;     // device address of the static scratch on this device
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured
+  // SB_OPT_RESTORE_FP32 under sb_graph_create: the eager run keeps its
+  // restored copy (restore_keep -> kept_*) for the graph, whose capture
+  // replays the restore into those blocks (graph_rest_*).
+  bool restore_keep = false;
+  void* kept_mem = nullptr;
+  void* kept_slots = nullptr;
+  size_t kept_mem_bytes = 0, kept_slots_bytes = 0;
+  void* graph_rest_mem = nullptr;
+  void* graph_rest_slots = nullptr;
+  size_t graph_rest_mem_bytes = 0, graph_rest_slots_bytes = 0;
 };
 
 struct sb_gather {
@@ -75,6 +86,7 @@ struct sb_gather {
   int64_t ncrs_slots = 0;
   int dm_azimuth_mdeg = -1;  // midpoint displacement of the slot tables (SB_OPT_DM_AZIMUTH at creation)
   std::vector<int32_t> cdp_off, crs_slot_off;  // host copies
+  std::vector<int32_t> crs_row_lo, crs_row_hi;  // rows [lo, hi) spanning each CDP's supergather
   size_t bytes = 0;
 };
 
@@ -89,6 +101,10 @@ struct sb_result {
 
 namespace sb {
 int ctx_alloc(sb_ctx* ctx, size_t bytes, void** p);
+// A block for work queued on the ctx stream only (a cached block is reused
+// with no host wait: the stream orders the new work after the old); no
+// error message when it does not fit (SB_ERR_BUDGET: the caller falls back).
+int ctx_alloc_transient(sb_ctx* ctx, size_t bytes, void** p);
 void ctx_free(sb_ctx* ctx, void* p);
 void ctx_release_all(sb_ctx* ctx);
 // Frees cached blocks, largest first, until at most keep bytes stay cached.

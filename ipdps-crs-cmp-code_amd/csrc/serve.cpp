@@ -106,8 +106,31 @@ size_t footprint(const sb_request& r, int ntr) {
   return b;
 }
 
-// Range request on a resident gather: only the results and the work table.
-size_t footprint_range(const sb_request& r) { return (size_t)(r.cdp_end - r.cdp_begin) * r.ns * 32 + (1 << 20); }
+// A 16-bit store's search restores the rows it reads to FP32 in a transient
+// copy (SB_OPT_RESTORE_FP32, on for the slot contexts): nrows rows.
+size_t restore_bytes(int store, int ns, int64_t nrows) {
+  if (store == SB_STORE_F32) return 0;
+  const int64_t stride = ((int64_t)ns + 8 + 63) / 64 * 64;
+  return (size_t)nrows * stride * 4 + 256;
+}
+
+// Range request on a resident gather: the results, the work table and a
+// 16-bit store's restored rows (CMP: the range's; CRS: its supergathers').
+size_t footprint_range(const sb_request& r, const sb_gather* g) {
+  const int cb = r.cdp_begin, ce = r.cdp_end;
+  int64_t rows = 0;
+  if (r.method == SB_METHOD_CMP) {
+    rows = g->cdp_off[ce] - g->cdp_off[cb];
+  } else {
+    int64_t lo = g->crs_row_lo[cb], hi = g->crs_row_hi[cb];
+    for (int i = cb + 1; i < ce; i++) {
+      lo = std::min<int64_t>(lo, g->crs_row_lo[i]);
+      hi = std::max<int64_t>(hi, g->crs_row_hi[i]);
+    }
+    rows = hi - lo;
+  }
+  return (size_t)(ce - cb) * r.ns * 32 + (1 << 20) + restore_bytes(g->store, g->ns, rows);
+}
 
 double evaluations_range(const sb_request& r, const sb_gather* g) {
   const int cb = r.cdp_begin, ce = r.cdp_end;
@@ -465,7 +488,7 @@ extern "C" int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* t
     q->r.samples = nullptr;
     q->r.nbr_off = q->r.nbr = nullptr;
     q->res = &R;
-    q->bytes = footprint_range(q->r);
+    q->bytes = footprint_range(q->r, R.g);
     q->evals = evaluations_range(q->r, R.g);
     R.users++;
     q->ticket = s->next_ticket++;
@@ -485,7 +508,7 @@ extern "C" int sb_server_submit(sb_server* s, const sb_request* req, uint64_t* t
   auto q = std::make_shared<Req>();
   q->r = r;
   q->ntr = r.cdp_off[r.ncdp];
-  q->bytes = footprint(r, q->ntr);
+  q->bytes = footprint(r, q->ntr) + restore_bytes(r.store, r.ns, q->ntr);
   q->evals = evaluations(r);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return fail(SB_ERR_STATE, "server is shutting down");

@@ -8,7 +8,7 @@
 // fastest again in interleaved rounds (so that a drift of the clock over the
 // first pass does not decide between candidates a few percent apart) and
 // keeps the one with the lowest median on the ctx; results are cached per
-// (method, store, w, ncdp, ns, npar, ntr).
+// (method, store, w, ncdp, ns, npar, ntr, SB_OPT_RESTORE_FP32).
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -70,7 +70,8 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
   if (!fast || !fast_supported(2 * tau + 1)) return SB_OK;
   const std::string key = std::to_string(method) + "/" + std::to_string(g->store) + "/" + std::to_string(w_key) + "/" +
                           std::to_string(g->ncdp) + "/" + std::to_string(g->ns) + "/" + std::to_string(npar) + "/" +
-                          std::to_string(g->ntr) + "/fast";
+                          std::to_string(g->ntr) + "/fast" +
+                          (g->store != SB_STORE_F32 && ctx->restore_fp32 ? "/restored" : "");
   auto hit = ctx->tune_cache.find(key);
   if (hit != ctx->tune_cache.end()) {
     ctx->fast_variant = hit->second.first;

@@ -58,11 +58,15 @@ def test_graph_replays_equal_eager(sb, gpu_ctx, persistent):
     print("case: graph replay persistent=%d" % persistent, flush=True)
     gpu_ctx.set_option(sb.OPT_PERSISTENT, persistent)
     g = _cmp_line(sb, gpu_ctx, sb.STORE_F32)
+    g16 = _cmp_line(sb, gpu_ctx, sb.STORE_F16)
     gc = _crs_line(sb, gpu_ctx, sb.STORE_F16)
     try:
+        # the 16-bit cases replay a restored FP32 copy held by the graph
+        # (SB_OPT_RESTORE_FP32), rewritten by every replay
         cases = [
             (g, sb.cmp_params(1.98e-7, 1.77e-6, 80, 0.002), None),
             (g, sb.cmp_params(1.98e-7, 1.77e-6, 33, 0.004, mode=sb.MODE_EXACT), (3, 11)),
+            (g16, sb.cmp_params(1.98e-7, 1.77e-6, 64, 0.002), (2, 9)),
             (gc, sb.crs_params(BOX[0], BOX[1], 4, BOX[2], BOX[3], 3, BOX[4], BOX[5], 5, 0.004), None),
             (gc, sb.de_params(*BOX, 0.004, pop=16, ngen=5, seed=3), (2, 6)),
         ]
@@ -87,4 +91,5 @@ def test_graph_replays_equal_eager(sb, gpu_ctx, persistent):
     finally:
         gpu_ctx.set_option(sb.OPT_PERSISTENT, 0)
         g.close()
+        g16.close()
         gc.close()
