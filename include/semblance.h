@@ -320,11 +320,27 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
  * restoring every window after its load.  Both evaluate the same FP32
  * values with the same FP32 arithmetic: the results are the same either
  * way.  No reference counterpart (the reference has no 16-bit store). */
+/* SB_OPT_DE_LDS: the CRS-DE fast search's window source.  -1 (default): the
+ * LDS-staged evaluator with 40 KiB per workgroup; 0: every window read
+ * from global memory; 4096..65536 (a multiple of 16): the LDS-staged
+ * evaluator with that many bytes of LDS per workgroup.  The workgroup walks
+ * the supergather in batches of consecutive traces whose window ranges over
+ * its t0 span and the (A, B, C) box it stages in LDS.  SB_OPT_DE_WAVES (4, 8
+ * = default, 16): waves per workgroup of the LDS-staged evaluator.
+ * SB_OPT_DE_PAIRS (1 = default): its slab holds every sample with its
+ * forward difference (8 bytes, one FMA a window sample); 0: samples only (4
+ * bytes, the difference formed per window).  Results
+ * are the same bits for every setting; a search the staging cannot serve (a
+ * 16-bit store read directly, ns too long for the slab) reads global memory.
+ * No reference counterpart (the reference ships no CRS-DE source). */
 enum {
   SB_OPT_PERSISTENT = 1,
   SB_OPT_STREAM_ROWS = 2, /* traces per sb_search_streamed batch, 0 = auto */
   SB_OPT_DM_AZIMUTH = 3,
-  SB_OPT_RESTORE_FP32 = 4
+  SB_OPT_RESTORE_FP32 = 4,
+  SB_OPT_DE_LDS = 5,
+  SB_OPT_DE_WAVES = 6,
+  SB_OPT_DE_PAIRS = 7
 };
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);

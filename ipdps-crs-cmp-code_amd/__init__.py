@@ -19,6 +19,9 @@ OPT_PERSISTENT = 1
 OPT_STREAM_ROWS = 2
 OPT_DM_AZIMUTH = 3
 OPT_RESTORE_FP32 = 4
+OPT_DE_LDS = 5
+OPT_DE_WAVES = 6
+OPT_DE_PAIRS = 7
 GRAPH_CRS_DE = 2
 ERR_DEADLINE = 9
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}

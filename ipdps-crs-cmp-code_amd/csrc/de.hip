@@ -324,6 +324,411 @@ __global__ __launch_bounds__(256) void k_crs_de(SearchArgs a, DeArgs d, LD ld) {
   }
 }
 
+// ----------------------------------------------------- LDS-staged CRS-DE
+// The fast evaluator with windows from LDS.  k_crs_de gives every
+// candidate of every trace its own scattered 16-byte window load, and the
+// vector-memory data path (TD ~0.99 busy) co-bounds it with its VALU work.
+// Here the workgroup -- NT = 64 NW / P2 consecutive t0 of one CDP, P2 lanes
+// (members) each -- walks the supergather in batches of consecutive traces
+// that all its candidates evaluate together:
+//
+// * Every trial stays in the (A, B, C) box (initial draws inside it, trials
+//   clipped to it), so a trace's window starts over the whole run are bounded
+//   by the box and the tile's t0 span: (T0 + a m)^2 + b m^2 + c h between
+//   its smallest and largest values over that box, square-rooted, widened by
+//   2 samples (the FP32 traveltime's rounding moves floor(t) by far less)
+//   and clipped to the valid starts [0, ns - w).  The bound holds for every
+//   generation, so the batches are planned once per workgroup: up to 64
+//   consecutive traces whose ranges, as 16-byte quads aligned as in the row,
+//   fit the slab; an LDS table keeps each trace's quads and window base.
+// * A trace whose unclipped bound lies inside the valid starts has no gated
+//   window for any candidate (and t^2 >= (tau + 2)^2 > 0), and a batch of
+//   such traces runs without the window test, the clamp before the square
+//   root and the per-trace count: 17 instead of 22 VALU a window.
+// * Each batch is staged by NW threads per trace (global loads through
+//   per-lane row pointers) as (sample, forward difference) pairs -- the FP32
+//   subtraction the trace loop would make; one FMA a window sample, W
+//   8-byte reads -- or as samples (PAIRS = false: 4 bytes, W + 1 reads, the
+//   difference formed per window); every lane then runs the batch's traces
+//   in order, the window at byte base + 8 floor(t) (or 4 floor(t)); gated
+//   windows read the zeros at address 0.
+//
+// The arithmetic, the trace order of every sum and the gating are those of
+// de_eval_fast (a paired slab holds the same difference, a gated window is
+// zero either way), so both kernels produce the same bits: same fitness,
+// same selections, same outputs.  FP32 rows only (16-bit stores run on their
+// restored FP32 copy, or on k_crs_de).
+namespace {
+template <class T>
+__device__ __forceinline__ T de_lds_at(unsigned addr) {
+  return *(__attribute__((address_space(3))) T*)(size_t)addr;
+}
+template <class T>
+__device__ __forceinline__ void de_lds_put(unsigned addr, T v) {
+  *(__attribute__((address_space(3))) T*)(size_t)addr = v;
+}
+}  // namespace
+
+// LDS layout of a workgroup over a supergather of S traces: 64 zero bytes;
+// the trace table (8 bytes a trace: the window base, then qlo | nq << 16 |
+// ungated << 31); the batch starts (u16, S + 2: starts with bit 15 set when
+// every trace of the batch is ungated, the end, and the batch count at index
+// S + 1); the slab of 32-byte quads of pairs.
+__host__ __device__ constexpr int de_lds_table(int S) { return 64; }
+__host__ __device__ constexpr int de_lds_batches(int S) { return 64 + 8 * S; }
+__host__ __device__ constexpr int de_lds_slab(int S) { return (64 + 8 * S + 2 * (S + 2) + 15) & ~15; }
+
+// Quads of trace k's window range over T0 in [T0lo, T0hi] and the box:
+// pairs of samples elo .. ehi + W - 1 as the row's quads qlo .. qlo + nq - 1
+// (nq = 0: no window of the span and box is valid).
+template <int W, bool PAIRS>
+__device__ __forceinline__ void de_range(const SearchArgs& a, const DeArgs& d, const SlotF& s, float T0lo,
+                                         float T0hi, int& qlo, int& nq, bool& full) {
+  const int nwin = max(a.ns - a.w, 0);
+  const float am0 = d.lo[0] * s.m, am1 = d.hi[0] * s.m;
+  const float ulo = T0lo + fminf(am0, am1), uhi = T0hi + fmaxf(am0, am1);
+  const float u2lo = ulo * ulo, u2hi = uhi * uhi;
+  const float usqmin = (ulo <= 0.f && uhi >= 0.f) ? 0.f : fminf(u2lo, u2hi);
+  const float usqmax = fmaxf(u2lo, u2hi);
+  const float bm0 = d.lo[1] * s.m2, bm1 = d.hi[1] * s.m2;
+  const float ch0 = d.lo[2] * s.h, ch1 = d.hi[2] * s.h;
+  const float ttlo = usqmin + fminf(bm0, bm1) + fminf(ch0, ch1);
+  const float tthi = usqmax + fmaxf(bm0, bm1) + fmaxf(ch0, ch1);
+  const int elo_b = (int)sqrtf(fmaxf(ttlo, 0.f)) - a.tau - 2;
+  const int ehi_b = (int)sqrtf(fmaxf(tthi, 0.f)) - a.tau + 2;
+  // every window of the span and box valid: no candidate is gated, and
+  // t^2 >= (tau + 2)^2 > 0 for all of them
+  full = elo_b >= 0 && ehi_b <= nwin - 1;
+  const int elo = max(elo_b, 0), ehi = min(ehi_b, nwin - 1);
+  qlo = 0;
+  nq = 0;
+  if (ehi >= elo) {
+    // pairs: samples elo .. ehi + W - 1 with their differences; samples:
+    // elo .. ehi + W
+    qlo = elo >> 2;
+    nq = ((ehi + W - (PAIRS ? 1 : 0)) >> 2) - qlo + 1;
+  }
+}
+
+template <int W, int NW, bool PAIRS>
+__global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) {
+  // as k_crs_de: the initial draws and the mutation are separate roundings
+  // (plain operators under this pragma: HIP's __fmul_rn / __fadd_rn are
+  // header functions whose operations contraction would still fuse); the
+  // trace loop spells out its FMAs
+#pragma clang fp contract(off)
+  constexpr int NT = 64 * NW;
+  const WorkItem wi = a.work[blockIdx.x];
+  if (wi.cdp < 0) return;  // the whole workgroup
+  const int cdp = wi.cdp;
+  const int P2 = d.pop <= 4 ? 4 : d.pop <= 8 ? 8 : d.pop <= 16 ? 16 : d.pop <= 32 ? 32 : 64;
+  const int per_block = NT / P2;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int seg = tid / P2;
+  const int member = tid % P2;
+  const int seg_base = (lane / P2) * P2;
+  const int t0first = wi.tile * per_block;
+  const int t0 = t0first + seg;
+  const bool live = member < d.pop && t0 < a.ns;
+  const int t0c = t0 < a.ns ? t0 : a.ns - 1;
+  const int k0 = a.slot_off[cdp];
+  const int S = a.slot_off[cdp + 1] - k0;
+  const float T0lo = (float)t0first, T0hi = (float)min(t0first + per_block - 1, a.ns - 1);
+  const unsigned tab = (unsigned)de_lds_table(S), bat = (unsigned)de_lds_batches(S), slab = (unsigned)de_lds_slab(S);
+  // slab bytes per staged sample and per quad
+  constexpr unsigned SB = PAIRS ? 8u : 4u, QB = 4u * SB;
+  constexpr int QSH = PAIRS ? 5 : 4;
+  const int capq = (a.lds_bytes - (int)slab) >> QSH;
+  const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
+  const SlotF* __restrict__ sl = a.slots + k0;
+
+  // ---- plan (once): each trace's quads, then the batches and window bases
+  if (tid < 16) de_lds_put<unsigned>(4u * tid, 0u);  // gated windows read these zeros
+  for (int k = tid; k < S; k += NT) {
+    int qlo, nq;
+    bool full;
+    de_range<W, PAIRS>(a, d, sl[k], T0lo, T0hi, qlo, nq, full);
+    de_lds_put<unsigned>(tab + 8u * k + 4u, (unsigned)qlo | ((unsigned)nq << 16) | (full ? 0x80000000u : 0u));
+  }
+  __syncthreads();
+  if (tid < 64) {
+    // one wave: the next up to 64 traces whose quads fit the slab (the
+    // inclusive sums are nondecreasing, so those that fit are a prefix)
+    int nb = 0;
+    for (int kb = 0; kb < S;) {
+      const int k = kb + lane;
+      const unsigned e = k < S ? de_lds_at<unsigned>(tab + 8u * k + 4u) : 0u;
+      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
+      int x = nq;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      int K = __builtin_amdgcn_readfirstlane(__popcll(__builtin_amdgcn_ballot_w64(k < S && x <= capq)));
+      K = K > 0 ? K : 1;  // the host sizes the slab for any single trace
+      if (lane < K) de_lds_put<unsigned>(tab + 8u * k, slab + QB * (unsigned)(x - nq - qlo) - SB * (unsigned)a.tau);
+      // bit 15: every trace of the batch has only valid windows
+      const bool gated = __builtin_amdgcn_ballot_w64(lane < K && !(e >> 31)) != 0ull;
+      if (lane == 0) de_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)(kb | (gated ? 0 : 0x8000)));
+      nb++;
+      kb += K;
+    }
+    if (lane == 0) {
+      de_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)S);
+      de_lds_put<unsigned short>(bat + 2u * (S + 1), (unsigned short)nb);
+    }
+  }
+  __syncthreads();
+  const int nbatch = __builtin_amdgcn_readfirstlane((int)de_lds_at<unsigned short>(bat + 2u * (S + 1)));
+
+  // Stage batch [kb, ke): trace kb + j by the NW threads j NW .. j NW + NW
+  // - 1, thread r taking the trace's quads r, r + NW, ...
+  auto stage = [&](int kb, int ke) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int j = tid / NW, r = tid % NW;
+    if (j < ke - kb) {
+      const int k = kb + j;
+      const unsigned base = de_lds_at<unsigned>(tab + 8u * k), e = de_lds_at<unsigned>(tab + 8u * k + 4u);
+      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
+      // base = slab + QB (qoff - qlo) - SB tau
+      const int qoff = ((int)(base - slab + SB * (unsigned)a.tau) >> QSH) + qlo;
+      // per-lane row pointers (a buffer resource would have to be uniform):
+      // quads up to the one holding sample ns - 1, then one sample more,
+      // inside the row's padding (stride >= ns + 8)
+      typedef const __attribute__((address_space(1))) float GF;
+      typedef const __attribute__((address_space(1))) f4 GF4;
+      GF* row = (GF*)(size_t)sl[k].addr;
+      for (int q = r; q < nq; q += NW) {
+        GF* p = row + 4 * (qlo + q);
+        const f4 v = *(GF4*)p;
+        const unsigned dst = slab + QB * (unsigned)(qoff + q);
+        if constexpr (PAIRS) {
+          const float nx = p[4];
+          de_lds_put<f4>(dst, f4{v.x, v.y - v.x, v.y, v.z - v.y});
+          de_lds_put<f4>(dst + 16u, f4{v.z, v.w - v.z, v.w, nx - v.w});
+        } else {
+          de_lds_put<f4>(dst, v);
+        }
+      }
+    }
+  };
+  // batch b's first trace (bit 15: the batch is ungated)
+  auto batch_at = [&](int b) { return __builtin_amdgcn_readfirstlane((int)de_lds_at<unsigned short>(bat + 2u * b)); };
+
+  const float T0 = (float)t0c;
+  const float T0sq = T0 * T0;
+  float num[W], den;
+  int cnt;
+  // One batch's traces [kb, ke) for this lane's candidate, in order, with
+  // de_eval_fast's arithmetic; lane j holds trace kb + j's window base.
+  // The window at LDS byte addr into the sums: W (sample, difference) pairs
+  // (8-byte aligned) or W + 1 samples (4-byte aligned: ds_read2_b32 pairs),
+  // interpolated as de_eval_fast interpolates.
+  auto accumulate = [&](unsigned addr, float x) {
+    if constexpr (PAIRS) {
+      typedef float f2 __attribute__((ext_vector_type(2)));
+      f2 pr[W];
+#pragma unroll
+      for (int q = 0; q < W; q++) pr[q] = de_lds_at<f2>(addr + 8u * q);
+#pragma unroll
+      for (int q = 0; q < W; q++) {
+        const float v = fmaf(pr[q].y, x, pr[q].x);
+        num[q] += v;
+        den = fmaf(v, v, den);
+      }
+    } else {
+      float r[W + 1];
+#pragma unroll
+      for (int q = 0; q <= W; q++) r[q] = de_lds_at<float>(addr + 4u * q);
+#pragma unroll
+      for (int q = 0; q < W; q++) {
+        const float v = fmaf(r[q + 1] - r[q], x, r[q]);
+        num[q] += v;
+        den = fmaf(v, v, den);
+      }
+    }
+  };
+  auto run_batch = [&](int kb, int ke, bool ungated, float K1, float K2, float pc) {
+    typedef const __attribute__((address_space(4))) SlotF CSlot;
+    const int K = ke - kb;
+    const unsigned mybase = lane < K ? de_lds_at<unsigned>(tab + 8u * (unsigned)(kb + lane)) : 0u;
+    CSlot* sk = (CSlot*)(size_t)(sl + kb);
+    if (ungated) {
+      // every window valid (the plan's bound): no test, no count per trace
+      for (int j = 0; j < K; j++) {
+        const float m = sk[j].m, m2 = sk[j].m2, h = sk[j].h;
+        const unsigned base = __builtin_amdgcn_readlane(mybase, j);
+        const float tt = fmaf(K2, m2, fmaf(K1, m, fmaf(pc, h, T0sq)));
+        const float ts = __builtin_amdgcn_sqrtf(tt);
+        const float x = __builtin_amdgcn_fractf(ts);
+        accumulate(base + ((unsigned)(int)ts << (PAIRS ? 3 : 2)), x);
+      }
+      cnt += K;
+      return;
+    }
+    for (int j = 0; j < K; j++) {
+      const float m = sk[j].m, m2 = sk[j].m2, h = sk[j].h;
+      const unsigned base = __builtin_amdgcn_readlane(mybase, j);
+      const float tt = fmaf(K2, m2, fmaf(K1, m, fmaf(pc, h, T0sq)));
+      const float ts = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.f));
+      const int it = (int)ts;
+      bool ok = (unsigned)(it - a.tau) < nwin;
+      if constexpr (W == 1) ok = ok && tt >= 0.f;
+      const float x = __builtin_amdgcn_fractf(ts);
+      const unsigned addr = ok ? base + ((unsigned)it << (PAIRS ? 3 : 2)) : 0u;
+      cnt += ok ? 1 : 0;
+      accumulate(addr, x);
+    }
+  };
+  // Fitness of (pa, pb, pc) over the whole supergather.  A supergather that
+  // is one batch is staged once for every generation.
+  const int b0 = batch_at(0);
+  if (nbatch == 1) {
+    stage(0, S);
+    __syncthreads();
+  }
+  auto fitness = [&](float pa, float pb, float pc, float& sem, float& stk) {
+    const float K1 = (2.f * T0) * pa, K2 = fmaf(pa, pa, pb);
+#pragma unroll
+    for (int q = 0; q < W; q++) num[q] = 0.f;
+    den = 0.f;
+    cnt = 0;
+    if (nbatch == 1) {
+      run_batch(0, S, (b0 & 0x8000) != 0, K1, K2, pc);
+    } else {
+      for (int b = 0; b < nbatch; b++) {
+        const int vb = batch_at(b), kb = vb & 0x7fff, ke = batch_at(b + 1) & 0x7fff;
+        __syncthreads();  // every lane is done with the previous batch's slab
+        stage(kb, ke);
+        __syncthreads();
+        run_batch(kb, ke, (vb & 0x8000) != 0, K1, K2, pc);
+      }
+    }
+    float acsq = 0.f, lin = 0.f;
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      acsq = fmaf(num[q], num[q], acsq);
+      lin += num[q];
+    }
+    const float mm = (float)cnt;
+    if (den > 1e-13f && cnt > 0 && S - cnt < 2) {
+      sem = acsq * __builtin_amdgcn_rcpf(den * mm);
+      stk = lin * __builtin_amdgcn_rcpf((float)W * mm);
+    } else {
+      sem = 0.f;
+      stk = 0.f;
+    }
+  };
+
+  float x[3], fit = 0.f, stk = 0.f;
+  const int mi = member < d.pop ? member : 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float u = de_unit(de_hash(d.seed, cdp, t0c, 0, mi, k));
+    const float span = d.hi[k] - d.lo[k];
+    const float us = u * span;
+    x[k] = d.lo[k] + us;
+  }
+  fitness(x[0], x[1], x[2], fit, stk);
+
+  for (int gen = 1; gen <= d.ngen; gen++) {
+    const int r1 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 0), d.pop, mi, 0, 0, 1);
+    const int r2 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 1), d.pop, mi, r1, 0, 2);
+    const int r3 = de_pick(de_hash(d.seed, cdp, t0c, gen, mi, 2), d.pop, mi, r1, r2, 3);
+    const int jr = (int)(de_hash(d.seed, cdp, t0c, gen, mi, 3) % 3u);
+    float tr[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float x1 = __shfl(x[k], seg_base + r1, 64);
+      const float x2 = __shfl(x[k], seg_base + r2, 64);
+      const float x3 = __shfl(x[k], seg_base + r3, 64);
+      const uint32_t u = de_hash(d.seed, cdp, t0c, gen, mi, 4 + k) >> 8;
+      if (u < d.cr_thr || k == jr) {
+        const float diff = x2 - x3;
+        const float fd = d.F * diff;
+        const float v = x1 + fd;
+        tr[k] = fminf(fmaxf(v, d.lo[k]), d.hi[k]);
+      } else {
+        tr[k] = x[k];
+      }
+    }
+    float tf, ts;
+    fitness(tr[0], tr[1], tr[2], tf, ts);
+    if (tf >= fit) {
+      x[0] = tr[0];
+      x[1] = tr[1];
+      x[2] = tr[2];
+      fit = tf;
+      stk = ts;
+    }
+  }
+
+  float bf = live ? fit : -1.f;
+  int bi = member;
+  for (int o = 1; o < P2; o <<= 1) {
+    const float of = __shfl_xor(bf, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (of > bf || (of == bf && oi < bi)) {
+      bf = of;
+      bi = oi;
+    }
+  }
+  const float ba = __shfl(x[0], seg_base + bi, 64);
+  const float bb = __shfl(x[1], seg_base + bi, 64);
+  const float bc = __shfl(x[2], seg_base + bi, 64);
+  const float bs = __shfl(stk, seg_base + bi, 64);
+  if (member == 0 && t0 < a.ns) {
+    const int64_t o = (int64_t)(cdp - a.cdp_base) * a.ns + t0;
+    a.out_index[o] = -1;
+    a.out_a[o] = ba;
+    a.out_b[o] = bb;
+    a.out_c[o] = (float)((double)bc / 1e6);
+    a.out_coher[o] = bf;
+    a.out_stack[o] = bs;
+  }
+}
+
+int de_lds_min_bytes(int ns, int smax, int pairs) {
+  // the table and batch starts of the longest supergather, then the largest
+  // single-trace range: every valid start plus the window, as quads, one
+  // more for the alignment
+  return de_lds_slab(smax) + (pairs ? 32 : 16) * ((ns + 3) / 4 + 2);
+}
+
+bool de_lds_supported(int w, int pop, int waves, int pairs, int lds_bytes, int ns, int smax) {
+  return de_supported(w, pop) && (w == 1 || w == 3 || w == 5 || w == 7) &&
+         (waves == 4 || waves == 8 || waves == 16) && 64 * waves >= pop && smax < 32768 &&
+         lds_bytes >= de_lds_min_bytes(ns, smax, pairs) && lds_bytes <= 64 * 1024;
+}
+
+template <int NW, bool PAIRS>
+static hipError_t de_lds_w(const SearchArgs& a, const DeArgs& d, int nblocks, hipStream_t st) {
+  const size_t sh = (size_t)a.lds_bytes;
+  switch (a.w) {
+    case 1: hipLaunchKernelGGL((k_crs_de_lds<1, NW, PAIRS>), dim3(nblocks), dim3(64 * NW), sh, st, a, d); break;
+    case 3: hipLaunchKernelGGL((k_crs_de_lds<3, NW, PAIRS>), dim3(nblocks), dim3(64 * NW), sh, st, a, d); break;
+    case 5: hipLaunchKernelGGL((k_crs_de_lds<5, NW, PAIRS>), dim3(nblocks), dim3(64 * NW), sh, st, a, d); break;
+    case 7: hipLaunchKernelGGL((k_crs_de_lds<7, NW, PAIRS>), dim3(nblocks), dim3(64 * NW), sh, st, a, d); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_de_lds(const SearchArgs& a, const DeArgs& d, int waves, int pairs, int smax, int nblocks,
+                         hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  if (!de_lds_supported(a.w, d.pop, waves, pairs, a.lds_bytes, a.ns, smax)) return hipErrorInvalidValue;
+  if (pairs) {
+    if (waves == 4) return de_lds_w<4, true>(a, d, nblocks, st);
+    if (waves == 8) return de_lds_w<8, true>(a, d, nblocks, st);
+    return de_lds_w<16, true>(a, d, nblocks, st);
+  }
+  if (waves == 4) return de_lds_w<4, false>(a, d, nblocks, st);
+  if (waves == 8) return de_lds_w<8, false>(a, d, nblocks, st);
+  return de_lds_w<16, false>(a, d, nblocks, st);
+}
+
 bool de_supported(int w, int pop) { return pop >= 4 && pop <= 64 && w >= 1 && w <= SB_MAX_W; }
 
 template <bool EXACT, int STORE, class LD>

@@ -104,5 +104,16 @@ hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);
 // CRS-DE (de.hip).  One work item = (cdp, tile of t0).
 bool de_supported(int w, int pop);
 hipError_t launch_de(const SearchArgs& a, const DeArgs& d, int store, int exact, int nblocks, hipStream_t st);
+// LDS-staged fast CRS-DE (FP32 rows; same results as launch_de's fast
+// kernel): workgroups of `waves` waves over 64 * waves / P2 consecutive t0
+// (P2 = the population rounded up to a power of two), a.lds_bytes of LDS
+// holding a 10-byte-a-trace plan of the supergather and the slab; smax =
+// the longest supergather searched.
+constexpr int kDeLdsDefaultBytes = 40 * 1024;
+// pairs: (sample, difference) slabs (one FMA a window sample) or samples.
+int de_lds_min_bytes(int ns, int smax, int pairs);
+bool de_lds_supported(int w, int pop, int waves, int pairs, int lds_bytes, int ns, int smax);
+hipError_t launch_de_lds(const SearchArgs& a, const DeArgs& d, int waves, int pairs, int smax, int nblocks,
+                         hipStream_t st);
 
 }  // namespace sb

@@ -1152,11 +1152,22 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
   d.lo[2] = (float)(p->c0 * 1e6);
   d.hi[2] = (float)(p->c1 * 1e6);
   const int P2 = d.pop <= 4 ? 4 : d.pop <= 8 ? 8 : d.pop <= 16 ? 16 : d.pop <= 32 ? 32 : 64;
-  const int per_block = 256 / P2;
-  const int ntiles = (g->ns + per_block - 1) / per_block;
 
   HIPR(hipSetDevice(ctx->device));
   std::lock_guard<std::mutex> lk(ctx->run_mu);
+  // A 16-bit store's fast search on a restored FP32 copy of its supergather
+  // rows, with a slot table addressing the copy (SB_OPT_RESTORE_FP32).
+  Restored rest;
+  const bool restored = de_mode == SB_MODE_FAST && rest.prepare(ctx, g, cb, ce, true, true);
+  // The LDS-staged evaluator (SB_OPT_DE_LDS) for fast searches of FP32 rows.
+  const int lds_bytes = ctx->de_lds < 0 ? kDeLdsDefaultBytes : ctx->de_lds;
+  int smax = 0;
+  for (int i = cb; i < ce; i++) smax = std::max(smax, g->crs_slot_off[i + 1] - g->crs_slot_off[i]);
+  const bool use_lds = de_mode == SB_MODE_FAST && ctx->de_lds != 0 && (restored || g->store == SB_STORE_F32) &&
+                       de_lds_supported(a.w, d.pop, ctx->de_waves, ctx->de_pairs, lds_bytes, g->ns, smax);
+  const int threads = use_lds ? 64 * ctx->de_waves : 256;
+  const int per_block = threads / P2;
+  const int ntiles = (g->ns + per_block - 1) / per_block;
   std::vector<int64_t> cost(g->ncdp);
   double tuples = 0;
   for (int i = 0; i < g->ncdp; i++) {
@@ -1168,10 +1179,6 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
     return rc;
   a.work = work_table(ctx);
   a.nwork = nblocks;
-  // A 16-bit store's fast search on a restored FP32 copy of its supergather
-  // rows, with a slot table addressing the copy (SB_OPT_RESTORE_FP32).
-  Restored rest;
-  const bool restored = de_mode == SB_MODE_FAST && rest.prepare(ctx, g, cb, ce, true, true);
   HIPR(record(ctx, 0));
   HIPR(record(ctx, 1));
   SearchArgs as = a;
@@ -1181,14 +1188,20 @@ extern "C" int sb_crs_de_run_range(sb_ctx* ctx, const sb_gather* g, const sb_de_
     e = rest.launch(g);
     if (e != hipSuccess) return hip_fail(e, "FP32 restore");
   }
-  e = launch_de(as, d, restored ? SB_STORE_F32 : g->store, de_mode == SB_MODE_EXACT, nblocks, ctx->stream);
+  if (use_lds) {
+    as.lds_bytes = lds_bytes;
+    e = launch_de_lds(as, d, ctx->de_waves, ctx->de_pairs, smax, nblocks, ctx->stream);
+  } else {
+    e = launch_de(as, d, restored ? SB_STORE_F32 : g->store, de_mode == SB_MODE_EXACT, nblocks, ctx->stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "CRS-DE launch");
   HIPR(record(ctx, 2));
   HIPR(record(ctx, 3));
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
-  ctx->last.block_threads = 256;
+  ctx->last.block_threads = threads;
+  ctx->last.lds_staged = use_lds ? 1 : 0;
   ctx->last.variant = -1;
   ctx->last.restored = restored ? 1 : 0;
   ctx->timing_pending = true;
@@ -1265,6 +1278,21 @@ extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
     ctx->restore_fp32 = value ? 1 : 0;
     return SB_OK;
   }
+  if (option == SB_OPT_DE_LDS) {
+    if (value != -1 && value != 0 && (value < 4096 || value > kLdsMaxBytes || value % 16))
+      return fail(SB_ERR_INVALID, "SB_OPT_DE_LDS must be -1, 0 or a multiple of 16 in [4096, 65536]");
+    ctx->de_lds = value;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DE_WAVES) {
+    if (value != 4 && value != 8 && value != 16) return fail(SB_ERR_INVALID, "SB_OPT_DE_WAVES must be 4, 8 or 16");
+    ctx->de_waves = value;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DE_PAIRS) {
+    ctx->de_pairs = value ? 1 : 0;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -1284,6 +1312,18 @@ extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
   }
   if (option == SB_OPT_RESTORE_FP32) {
     *value = ctx->restore_fp32;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DE_LDS) {
+    *value = ctx->de_lds;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DE_WAVES) {
+    *value = ctx->de_waves;
+    return SB_OK;
+  }
+  if (option == SB_OPT_DE_PAIRS) {
+    *value = ctx->de_pairs;
     return SB_OK;
   }
   return fail(SB_ERR_INVALID, "unknown option");

@@ -50,6 +50,9 @@ struct sb_ctx {
   int stream_rows = 0;         // SB_OPT_STREAM_ROWS (0: automatic)
   int dm_azimuth_mdeg = -1;    // SB_OPT_DM_AZIMUTH (-1: the reference's unsigned distance)
   int restore_fp32 = 1;        // SB_OPT_RESTORE_FP32
+  int de_lds = -1;             // SB_OPT_DE_LDS (-1: automatic slab size, 0: global-memory evaluator)
+  int de_waves = 8;            // SB_OPT_DE_WAVES
+  int de_pairs = 1;            // SB_OPT_DE_PAIRS
   void* scratch = nullptr;     // device address of the static scratch on this device
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured
