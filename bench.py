@@ -308,38 +308,38 @@ def _crs_reference_block(sb, cdp_off, geom, samples, n):
                     sample=sample + "; reference INFO line")
 
 
-def de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, cdps):
-    """The oracle's CPU twin of CRS-DE (OpenMP, FP32) on the given CDPs."""
+def de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, every=250):
+    """The oracle's CPU twin of CRS-DE (OpenMP, FP32) on a stratified sample:
+    every `every`-th CDP (from every/2), each at its full supergather and ns."""
     cores = len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(cores)
-    # restrict to the CDPs' neighbourhoods: keep every CDP, search all, time per tuple
     c = DE_CFG
-    lo = min(cdps)
-    hi = max(cdps) + 1
-    sub_off = (cdp_off[lo:hi + 1] - cdp_off[lo]).astype(np.int32)
-    rows = np.arange(cdp_off[lo], cdp_off[hi])
-    # neighbour lists of the subrange, clipped to it
-    no, nb = [0], []
-    for i in range(lo, hi):
-        lst = [j - lo for j in nbr[nbr_off[i]:nbr_off[i + 1]] if lo <= j < hi]
-        nb += lst
-        no.append(len(nb))
-    no, nb = np.array(no, np.int32), np.array(nb, np.int32)
-    folds = np.diff(sub_off)
-    tuples = sum(float(sum(folds[j] for j in nb[no[i]:no[i + 1]])) for i in range(hi - lo)) * c["ns"] * c["pop"] * (
-        c["ngen"] + 1)
+    ncdp = len(cdp_off) - 1
+    cdps = list(range(every // 2, ncdp, every))
+    folds = np.diff(cdp_off)
+    sizes = [int(sum(folds[j] for j in nbr[nbr_off[i]:nbr_off[i + 1]])) for i in cdps]
+    tuples = float(sum(sizes)) * c["ns"] * c["pop"] * (c["ngen"] + 1)
     t = time.perf_counter()
-    orc.crs_de(sub_off, geom[rows], samples[rows], c["dt_us"], no, nb, BOX["a0"], BOX["a1"], BOX["b0"], BOX["b1"],
-               BOX["c0"], BOX["c1"], c["tau"], c["pop"], c["ngen"], c["F"], c["CR"], 1)
+    for i in cdps:
+        orc.crs_de(cdp_off, geom, samples, c["dt_us"], nbr_off, nbr, BOX["a0"], BOX["a1"], BOX["b0"], BOX["b1"],
+                   BOX["c0"], BOX["c1"], c["tau"], c["pop"], c["ngen"], c["F"], c["CR"], 1, cdp_range=(i, i + 1))
     wall = time.perf_counter() - t
     return dict(value=tuples / wall, unit="semblance evals/s", cores=cores, kind="port",
-                sample="CDPs %d-%d (%.3g evals), oracle CPU twin of CRS-DE, FP32" % (lo, hi - 1, tuples))
+                sample="every %dth CDP (%d CDPs, supergathers of %d-%d traces, %.3g evals) at full ns, oracle CPU "
+                       "twin of CRS-DE, FP32" % (every, len(cdps), min(sizes), max(sizes), tuples))
 
 
 def side_configs(sb, args):
     """CRS grid (500 CDPs) and CRS-DE (2000 CDPs) at their full sizes."""
-    out = {}
     orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+    out = crs_grid_side(sb, args, orc)
+    out.update(crs_de_side(sb, args, orc))
+    return out
+
+
+def crs_grid_side(sb, args, orc):
+    """The CRS grid configuration: 500 CDPs, 11^3 grid."""
+    out = {}
     print("bench: generating CRS grid workload", flush=True)
     w = crs_workload(sb, CRS_CFG, SEED + 2)
     cdp_off, geom, samples, nbr_off, nbr = w
@@ -361,14 +361,23 @@ def side_configs(sb, args):
     if not args.no_cpu:
         print("bench: CRS CPU baseline", flush=True)
         out["crs_grid"]["cpu_baseline"] = crs_cpu_baseline(sb, orc, cdp_off, geom, samples, nbr_off, nbr)
-    del w, cdp_off, geom, samples
+    return out
+
+
+def crs_de_side(sb, args, orc, ncdp=2000, stores=("f16", "f32"), lds=-1):
+    """The CRS-DE configuration (the first ncdp CDPs of its line): each store
+    on the default evaluator (lds = SB_OPT_DE_LDS), FP32 also on the
+    global-memory one, 16-bit stores also read directly."""
+    out = {}
+    b = BOX
     print("bench: generating CRS-DE workload", flush=True)
-    cdp_off, geom, samples, nbr_off, nbr = crs_workload(sb, DE_CFG, SEED + 3)
+    cdp_off, geom, samples, nbr_off, nbr = crs_workload(sb, dict(DE_CFG, ncdp=ncdp), SEED + 3)
     c = DE_CFG
-    out["crs_de"] = dict(workload="CRS-DE 2000 CDPs fold 200-1000 (aph 600 -> %d traces) ns 1001 pop 32 ngen 30" %
-                         len(geom))
+    out["crs_de"] = dict(workload="CRS-DE %d CDPs fold 200-1000 (aph 600 -> %d traces) ns 1001 pop 32 ngen 30" %
+                         (ncdp, len(geom)))
     with sb.Context(0) as ctx:
-        for store in ("f16", "f32"):
+        ctx.set_option(sb.OPT_DE_LDS, lds)
+        for store in stores:
             g = sb.Gather(ctx, cdp_off, geom, samples, c["dt_us"], store=sb.STORES[store], nbr_off=nbr_off, nbr=nbr)
             prm = sb.de_params(b["a0"], b["a1"], b["b0"], b["b1"], b["c0"], b["c1"], c["tau"], c["pop"], c["ngen"],
                                c["F"], c["CR"], 1, sb.MODE_FAST)
@@ -377,9 +386,23 @@ def side_configs(sb, args):
             tuples = tk[0]["tuples"]
             out["crs_de"][store] = dict(value=tuples * 2 / wall, ms_per_step=wall / 2 * 1e3, tuples=tuples,
                                         search_ms=ms, roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms * 1e-3) /
-                                        1e12 / PEAK_FP32_VECTOR_TFLOPS, restored=tk[0]["restored"])
-            print("bench: crs-de %s %.4g evals/s (%.1f ms)" % (store, out["crs_de"][store]["value"], ms), flush=True)
-            if store != "f32":
+                                        1e12 / PEAK_FP32_VECTOR_TFLOPS, restored=tk[0]["restored"],
+                                        lds_staged=tk[0]["lds_staged"], block_threads=tk[0]["block_threads"])
+            print("bench: crs-de %s %.4g evals/s (%.1f ms, LDS-staged %d)" %
+                  (store, out["crs_de"][store]["value"], ms, tk[0]["lds_staged"]), flush=True)
+            if store == "f32" and lds != 0 and not args.de_only:
+                # the global-memory evaluator (every window its own load) on the same step
+                ctx.set_option(sb.OPT_DE_LDS, 0)
+                try:
+                    wall3, tk3 = timed_steps(sb, ctx, g, prm, 2, 1)
+                finally:
+                    ctx.set_option(sb.OPT_DE_LDS, lds)
+                ms3 = float(np.mean([t["search_ms"] for t in tk3]))
+                out["crs_de"]["f32_global"] = dict(value=tuples * 2 / wall3, search_ms=ms3,
+                                                   roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms3 * 1e-3) / 1e12 /
+                                                   PEAK_FP32_VECTOR_TFLOPS)
+                print("bench: crs-de f32 global-memory evaluator %.1f ms" % ms3, flush=True)
+            if store != "f32" and not args.de_only:
                 ctx.set_option(sb.OPT_RESTORE_FP32, 0)
                 try:
                     wall2, tk2 = timed_steps(sb, ctx, g, prm, 2, 1)
@@ -392,10 +415,12 @@ def side_configs(sb, args):
                       (store, out["crs_de"][store + "_direct"]["value"], ms2), flush=True)
             g.close()
     if not args.no_cpu:
-        cpu = de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr, [100, 101, 102, 103])
+        cpu = de_cpu_baseline(orc, cdp_off, geom, samples, nbr_off, nbr)
         out["crs_de"]["cpu_baseline"] = cpu
-        out["crs_de"]["f16_vs_cpu"] = out["crs_de"]["f16"]["value"] / cpu["value"]
-    out["crs_de"]["f16_vs_f32"] = out["crs_de"]["f16"]["value"] / out["crs_de"]["f32"]["value"]
+        if "f16" in out["crs_de"]:
+            out["crs_de"]["f16_vs_cpu"] = out["crs_de"]["f16"]["value"] / cpu["value"]
+    if "f16" in out["crs_de"] and "f32" in out["crs_de"]:
+        out["crs_de"]["f16_vs_f32"] = out["crs_de"]["f16"]["value"] / out["crs_de"]["f32"]["value"]
     return out
 
 
@@ -580,12 +605,24 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop load test")
     ap.add_argument("--no-modes", action="store_true", help="skip the persistent / LDS / graph launch-mode timings")
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep over arrival rates")
+    ap.add_argument("--de-only", action="store_true",
+                    help="time only the CRS-DE configuration (for profiling): --de-cdps, --de-lds, --stores")
+    ap.add_argument("--de-cdps", type=int, default=2000, help="with --de-only: CDPs of the CRS-DE line")
+    ap.add_argument("--de-lds", type=int, default=-1, help="with --de-only: SB_OPT_DE_LDS (0: global memory)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
               "with no exchange, see the sharding notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     sb = _load("sbamd", os.path.join(ROOT, "ipdps-crs-cmp-code_amd", "__init__.py"))
+    if args.de_only:
+        orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+        if args.no_cpu:
+            print(json.dumps(crs_de_side(sb, args, orc, args.de_cdps, tuple(args.stores.split(",")), args.de_lds)),
+                  flush=True)
+            return
+        print("bench.py: --de-only times the GPU only; pass --no-cpu", file=sys.stderr)
+        sys.exit(2)
     print("bench: generating synthetic line", flush=True)
     cdp_off, geom, samples = sb.synth(0, CFG["ncdp"], CFG["fold"], CFG["ns"], CFG["dt_us"], SEED + 1,
                                       h_max=CFG["h_max"], h_jitter=CFG["h_jitter"], nevents=CFG["nevents"],
