@@ -320,6 +320,20 @@ int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* params, int
  * restoring every window after its load.  Both evaluate the same FP32
  * values with the same FP32 arithmetic: the results are the same either
  * way.  No reference counterpart (the reference has no 16-bit store). */
+/* SB_OPT_CRS_LDS: the CRS grid fast search's window source (FP32 rows and
+ * 16-bit stores' restored copies).  -1 (default): the LDS-staged search with
+ * 40 KiB per workgroup; 0: every window read from global memory (the launch
+ * shape sb_tune / sb_tune_set choose); 4096..65536 (a multiple of 16): the
+ * LDS-staged search with that many bytes of LDS per workgroup.  A workgroup
+ * covers one (CDP, 64-t0 tile) and several (a, b) pairs of the grid with
+ * their c values, and walks the supergather in batches of consecutive
+ * traces whose window ranges over its parameter box it stages in LDS.
+ * SB_OPT_CRS_PAIRS (1 = default): the slab holds (sample, difference)
+ * pairs; 0: samples.  SB_OPT_CRS_CPW: c values each wave holds (6 or 12; 0 =
+ * default: 6).  Results are the same bits for every setting; a search
+ * the staging cannot serve (more than 48 c values, ns and the longest
+ * supergather too large for the slab) reads global memory.  No reference
+ * counterpart. */
 /* SB_OPT_DE_LDS: the CRS-DE fast search's window source.  -1 (default): the
  * LDS-staged evaluator with 40 KiB per workgroup; 0: every window read
  * from global memory; 4096..65536 (a multiple of 16): the LDS-staged
@@ -340,7 +354,10 @@ enum {
   SB_OPT_RESTORE_FP32 = 4,
   SB_OPT_DE_LDS = 5,
   SB_OPT_DE_WAVES = 6,
-  SB_OPT_DE_PAIRS = 7
+  SB_OPT_DE_PAIRS = 7,
+  SB_OPT_CRS_LDS = 8,
+  SB_OPT_CRS_PAIRS = 9,
+  SB_OPT_CRS_CPW = 10
 };
 int sb_ctx_set_option(sb_ctx* ctx, int option, int value);
 int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value);

@@ -22,6 +22,9 @@ OPT_RESTORE_FP32 = 4
 OPT_DE_LDS = 5
 OPT_DE_WAVES = 6
 OPT_DE_PAIRS = 7
+OPT_CRS_LDS = 8
+OPT_CRS_PAIRS = 9
+OPT_CRS_CPW = 10
 GRAPH_CRS_DE = 2
 ERR_DEADLINE = 9
 STORES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}

@@ -20,7 +20,7 @@ OBJ = os.path.join(LIB, "obj")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["su_io.cpp", "synth.cpp", "runtime.cpp", "tune.cpp", "serve.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "de.hip", "lds.hip", "probe.hip"]
+DEVICE_SOURCES = ["kernels.hip", "de.hip", "lds.hip", "crs_lds.hip", "probe.hip"]
 CLI_SOURCES = {"cmp": "cli_cmp.cpp", "crs": "cli_crs.cpp", "crs-de": "cli_crs_de.cpp"}
 
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + os.path.join(ROCM, "include")]

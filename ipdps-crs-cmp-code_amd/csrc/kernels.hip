@@ -667,10 +667,12 @@ __device__ __forceinline__ void search_item(const SearchArgs& a, const WorkItem 
       act[ci] = q < pw1;
       const int qq = act[ci] ? q : p;
       if constexpr (CRS) {
+        // a0 + inc i as one FMA each (what the compiler made of the
+        // separate operations; k_search_crs_lds writes the same)
         const int ia = qq / nbnc, ib = (qq / a.nc) % a.nb, ic = qq % a.nc;
-        pa[ci] = __fadd_rn(a.fa0, __fmul_rn(a.finc_a, (float)ia));
-        pb[ci] = __fadd_rn(a.fb0, __fmul_rn(a.finc_b, (float)ib));
-        pc[ci] = __fadd_rn(a.fc0, __fmul_rn(a.finc_c, (float)ic));
+        pa[ci] = fmaf(a.finc_a, (float)ia, a.fa0);
+        pb[ci] = fmaf(a.finc_b, (float)ib, a.fb0);
+        pc[ci] = fmaf(a.finc_c, (float)ic, a.fc0);
       } else {
         pa[ci] = 0.f;
         pb[ci] = 0.f;

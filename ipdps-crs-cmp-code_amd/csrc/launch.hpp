@@ -104,6 +104,19 @@ hipError_t launch_finalize(const SearchArgs& a, int store, hipStream_t st);
 // CRS-DE (de.hip).  One work item = (cdp, tile of t0).
 bool de_supported(int w, int pop);
 hipError_t launch_de(const SearchArgs& a, const DeArgs& d, int store, int exact, int nblocks, hipStream_t st);
+// LDS-staged CRS grid search (FP32 rows; same keys as launch_search_fast's
+// CRS kernel): workgroups of `waves` waves over (CDP, 64-t0 tile) and
+// crs_lds_pairs_per_block (a, b) pairs of the grid (work items' parameter
+// ranges cover whole pairs), a.lds_bytes of LDS for the plan and the slab;
+// smax = the longest supergather searched.
+constexpr int kCrsLdsDefaultBytes = 40 * 1024;
+constexpr int kCrsLdsWaves = 8;
+// cpw: c values a wave holds (6 or 12; 0: automatic).
+int crs_lds_min_bytes(int ns, int smax, int pairs, int waves);
+bool crs_lds_supported(int w, int nc, int cpw, int waves, int pairs, int lds_bytes, int ns, int smax);
+int crs_lds_pairs_per_block(int nc, int cpw, int waves);
+hipError_t launch_search_crs_lds(const SearchArgs& a, int cpw, int waves, int pairs, int smax, int nblocks,
+                                 hipStream_t st);
 // LDS-staged fast CRS-DE (FP32 rows; same results as launch_de's fast
 // kernel): workgroups of `waves` waves over 64 * waves / P2 consecutive t0
 // (P2 = the population rounded up to a power of two), a.lds_bytes of LDS

@@ -825,6 +825,8 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   for (int i = cb; i < ce; i++) tuples += (double)S[i] * g->ns * params_per_tuple;
   int variant = -1;
   bool lds = false;
+  bool crs_lds = false;  // the LDS-staged CRS grid search
+  int smax = 0;
   FastVariant v{};
   // Windows the fast kernel is not compiled for (w > 7) run in reference
   // arithmetic: slower, same parity bar.
@@ -865,6 +867,19 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
     pchunk = ctx->pchunk_override > 0 ? ctx->pchunk_override
              : lds ? lds_default_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC)
                    : choose_pchunk(ctx, ce - cb, ntiles, a.npar, v.waves * v.CC);
+    // CRS on FP32 rows: the LDS-staged search (SB_OPT_CRS_LDS), its work items
+    // (CDP, 64-t0 tile, whole (a, b) pairs of the grid)
+    if (a.method == SB_METHOD_CRS && ctx->crs_lds != 0 && kstore == SB_STORE_F32 && !sc.base) {
+      const int lb = ctx->crs_lds < 0 ? kCrsLdsDefaultBytes : ctx->crs_lds;
+      for (int i = cb; i < ce; i++) smax = std::max<int>(smax, (int)S[i]);
+      if (crs_lds_supported(a.w, a.nc, ctx->crs_cpw, kCrsLdsWaves, ctx->crs_pairs, lb, g->ns, smax)) {
+        crs_lds = true;
+        a.lds_bytes = lb;
+        family = 3;
+        ntiles = (g->ns + 63) / 64;
+        pchunk = crs_lds_pairs_per_block(a.nc, ctx->crs_cpw, kCrsLdsWaves) * a.nc;
+      }
+    }
   } else if (tiled_exact) {
     family = 2;
     ntiles = (g->ns + 63) / 64;
@@ -902,7 +917,7 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   hipError_t e = hipSuccess;
   if (mode == SB_MODE_FAST || tiled_exact) e = launch_fill_u64(a.keys, nout, 0ull, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fill");
-  const bool persistent = mode == SB_MODE_FAST && !lds && ctx->persistent && !sc.base;
+  const bool persistent = mode == SB_MODE_FAST && !lds && !crs_lds && ctx->persistent && !sc.base;
   if (persistent) {
     // Persistent grid: 8 per-XCD item counters, cleared by a kernel (not a
     // memset, so that the sequence can be captured into a graph).
@@ -918,7 +933,9 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
     e = rest.launch(g);
     if (e != hipSuccess) return hip_fail(e, "FP32 restore");
   }
-  if (mode == SB_MODE_FAST && lds)
+  if (mode == SB_MODE_FAST && crs_lds)
+    e = launch_search_crs_lds(as, ctx->crs_cpw, kCrsLdsWaves, ctx->crs_pairs, smax, nblocks, ctx->stream);
+  else if (mode == SB_MODE_FAST && lds)
     e = launch_search_lds(as, kstore, v, nblocks, ctx->stream);
   else if (mode == SB_MODE_FAST)
     e = launch_search_fast(as, kstore, variant, nblocks, ctx->num_cus, ctx->stream);
@@ -936,9 +953,10 @@ int run_search(sb_ctx* ctx, const sb_gather* g, SearchArgs& a, int mode, double 
   ctx->last = sb_timing{};
   ctx->last.tuples = tuples;
   ctx->last.grid_blocks = nblocks;
-  ctx->last.block_threads = mode == SB_MODE_FAST || tiled_exact ? (lds ? 64 * v.waves : 256) : 64;
-  ctx->last.variant = variant;
-  ctx->last.lds_staged = lds ? 1 : 0;
+  ctx->last.block_threads =
+      mode == SB_MODE_FAST || tiled_exact ? (crs_lds ? 64 * kCrsLdsWaves : lds ? 64 * v.waves : 256) : 64;
+  ctx->last.variant = crs_lds ? -2 : variant;
+  ctx->last.lds_staged = lds || crs_lds ? 1 : 0;
   ctx->last.restored = restored ? 1 : 0;
   ctx->timing_pending = true;
   ctx->timing_graph = false;
@@ -1293,6 +1311,21 @@ extern "C" int sb_ctx_set_option(sb_ctx* ctx, int option, int value) {
     ctx->de_pairs = value ? 1 : 0;
     return SB_OK;
   }
+  if (option == SB_OPT_CRS_LDS) {
+    if (value != -1 && value != 0 && (value < 4096 || value > kLdsMaxBytes || value % 16))
+      return fail(SB_ERR_INVALID, "SB_OPT_CRS_LDS must be -1, 0 or a multiple of 16 in [4096, 65536]");
+    ctx->crs_lds = value;
+    return SB_OK;
+  }
+  if (option == SB_OPT_CRS_PAIRS) {
+    ctx->crs_pairs = value ? 1 : 0;
+    return SB_OK;
+  }
+  if (option == SB_OPT_CRS_CPW) {
+    if (value != 0 && value != 6 && value != 12) return fail(SB_ERR_INVALID, "SB_OPT_CRS_CPW must be 0, 6 or 12");
+    ctx->crs_cpw = value;
+    return SB_OK;
+  }
   return fail(SB_ERR_INVALID, "unknown option");
 }
 
@@ -1324,6 +1357,18 @@ extern "C" int sb_ctx_get_option(const sb_ctx* ctx, int option, int* value) {
   }
   if (option == SB_OPT_DE_PAIRS) {
     *value = ctx->de_pairs;
+    return SB_OK;
+  }
+  if (option == SB_OPT_CRS_LDS) {
+    *value = ctx->crs_lds;
+    return SB_OK;
+  }
+  if (option == SB_OPT_CRS_PAIRS) {
+    *value = ctx->crs_pairs;
+    return SB_OK;
+  }
+  if (option == SB_OPT_CRS_CPW) {
+    *value = ctx->crs_cpw;
     return SB_OK;
   }
   return fail(SB_ERR_INVALID, "unknown option");

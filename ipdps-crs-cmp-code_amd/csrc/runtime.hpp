@@ -53,6 +53,9 @@ struct sb_ctx {
   int de_lds = -1;             // SB_OPT_DE_LDS (-1: automatic slab size, 0: global-memory evaluator)
   int de_waves = 8;            // SB_OPT_DE_WAVES
   int de_pairs = 1;            // SB_OPT_DE_PAIRS
+  int crs_lds = -1;            // SB_OPT_CRS_LDS (-1: automatic slab size, 0: global-memory search)
+  int crs_pairs = 1;           // SB_OPT_CRS_PAIRS
+  int crs_cpw = 0;             // SB_OPT_CRS_CPW (0: automatic)
   void* scratch = nullptr;     // device address of the static scratch on this device
   bool capturing = false;      // a graph is being captured on `stream`
   void* graph_work = nullptr;  // work table copy owned by the graph being captured

@@ -68,6 +68,16 @@ extern "C" int sb_tune(sb_ctx* ctx, const sb_gather* g, int method, const void* 
   // 7 samples, CMP grids with c < 0) do not use the tuned shape: nothing to
   // time, and nothing cached under a key that fast searches would reuse.
   if (!fast || !fast_supported(2 * tau + 1)) return SB_OK;
+  // CRS searches the LDS-staged kernel serves (SB_OPT_CRS_LDS) have no
+  // launch shape to choose: their work items are fixed by the grid.
+  if (method == SB_METHOD_CRS && ctx->crs_lds != 0 && (g->store == SB_STORE_F32 || ctx->restore_fp32)) {
+    const sb_crs_params* p = (const sb_crs_params*)params;
+    int smax = 0;
+    for (int i = 0; i < g->ncdp; i++) smax = std::max(smax, g->crs_slot_off[i + 1] - g->crs_slot_off[i]);
+    const int lb = ctx->crs_lds < 0 ? kCrsLdsDefaultBytes : ctx->crs_lds;
+    if (crs_lds_supported(2 * tau + 1, p->nc, ctx->crs_cpw, kCrsLdsWaves, ctx->crs_pairs, lb, g->ns, smax))
+      return SB_OK;
+  }
   const std::string key = std::to_string(method) + "/" + std::to_string(g->store) + "/" + std::to_string(w_key) + "/" +
                           std::to_string(g->ncdp) + "/" + std::to_string(g->ns) + "/" + std::to_string(npar) + "/" +
                           std::to_string(g->ntr) + "/fast" +
