@@ -337,26 +337,44 @@ def side_configs(sb, args):
     return out
 
 
-def crs_grid_side(sb, args, orc):
-    """The CRS grid configuration: 500 CDPs, 11^3 grid."""
+def crs_grid_side(sb, args, orc, ncdp=500, lds=-1):
+    """The CRS grid configuration (the first ncdp CDPs of its line), 11^3
+    grid: the default search (lds = SB_OPT_CRS_LDS), and the global-memory
+    search at its tuned launch shape beside it."""
     out = {}
     print("bench: generating CRS grid workload", flush=True)
-    w = crs_workload(sb, CRS_CFG, SEED + 2)
+    w = crs_workload(sb, dict(CRS_CFG, ncdp=ncdp), SEED + 2)
     cdp_off, geom, samples, nbr_off, nbr = w
     b = BOX
     prm = sb.crs_params(b["a0"], b["a1"], 11, b["b0"], b["b1"], 11, b["c0"], b["c1"], 11, CRS_CFG["tau"],
                         sb.MODE_FAST)
     with sb.Context(0) as ctx:
+        ctx.set_option(sb.OPT_CRS_LDS, lds)
         g = sb.Gather(ctx, cdp_off, geom, samples, CRS_CFG["dt_us"], nbr_off=nbr_off, nbr=nbr)
         if not args.no_tune:
             sb.tune(ctx, g, prm, reps=1)
         wall, tk = timed_steps(sb, ctx, g, prm, 3, 1)
         ms = float(np.mean([t["search_ms"] for t in tk]))
         tuples = tk[0]["tuples"]
-        out["crs_grid"] = dict(workload="CRS 500 CDPs fold 1000 (aph 600 -> %d traces) ns 1001 11x11x11 apm 50 FP32" %
-                               len(geom), value=tuples * 3 / wall, ms_per_step=wall / 3 * 1e3, tuples=tuples,
+        out["crs_grid"] = dict(workload="CRS %d CDPs fold 1000 (aph 600 -> %d traces) ns 1001 11x11x11 apm 50 FP32" %
+                               (ncdp, len(geom)), value=tuples * 3 / wall, ms_per_step=wall / 3 * 1e3, tuples=tuples,
                                search_ms=ms, roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms * 1e-3) / 1e12 /
-                               PEAK_FP32_VECTOR_TFLOPS)
+                               PEAK_FP32_VECTOR_TFLOPS, lds_staged=tk[0]["lds_staged"])
+        print("bench: crs grid %.1f ms (LDS-staged %d)" % (ms, tk[0]["lds_staged"]), flush=True)
+        if lds != 0 and not args.crs_only:
+            # the global-memory search (one window load per evaluation) at its tuned shape
+            ctx.set_option(sb.OPT_CRS_LDS, 0)
+            try:
+                if not args.no_tune:
+                    sb.tune(ctx, g, prm, reps=1)
+                wall2, tk2 = timed_steps(sb, ctx, g, prm, 3, 1)
+            finally:
+                ctx.set_option(sb.OPT_CRS_LDS, lds)
+            ms2 = float(np.mean([t["search_ms"] for t in tk2]))
+            out["crs_grid"]["global"] = dict(value=tuples * 3 / wall2, search_ms=ms2,
+                                             roofline_frac=tuples * FLOP_PER_TUPLE_CRS / (ms2 * 1e-3) / 1e12 /
+                                             PEAK_FP32_VECTOR_TFLOPS, variant=tk2[0]["variant"])
+            print("bench: crs grid global-memory search %.1f ms" % ms2, flush=True)
         g.close()
     if not args.no_cpu:
         print("bench: CRS CPU baseline", flush=True)
@@ -609,12 +627,23 @@ def main():
                     help="time only the CRS-DE configuration (for profiling): --de-cdps, --de-lds, --stores")
     ap.add_argument("--de-cdps", type=int, default=2000, help="with --de-only: CDPs of the CRS-DE line")
     ap.add_argument("--de-lds", type=int, default=-1, help="with --de-only: SB_OPT_DE_LDS (0: global memory)")
+    ap.add_argument("--crs-only", action="store_true",
+                    help="time only the CRS grid configuration (for profiling): --crs-cdps, --crs-lds")
+    ap.add_argument("--crs-cdps", type=int, default=500, help="with --crs-only: CDPs of the CRS line")
+    ap.add_argument("--crs-lds", type=int, default=-1, help="with --crs-only: SB_OPT_CRS_LDS (0: global memory)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
               "with no exchange, see the sharding notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     sb = _load("sbamd", os.path.join(ROOT, "ipdps-crs-cmp-code_amd", "__init__.py"))
+    if args.crs_only:
+        orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+        if args.no_cpu:
+            print(json.dumps(crs_grid_side(sb, args, orc, args.crs_cdps, args.crs_lds)), flush=True)
+            return
+        print("bench.py: --crs-only times the GPU only; pass --no-cpu", file=sys.stderr)
+        sys.exit(2)
     if args.de_only:
         orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
         if args.no_cpu:
