@@ -38,65 +38,15 @@
 #include "device_types.hpp"
 #include "launch.hpp"
 #include "semblance.h"
+#include "staged.hpp"
 
 namespace sb {
-
-namespace {
-template <class T>
-__device__ __forceinline__ T crs_lds_at(unsigned addr) {
-  return *(__attribute__((address_space(3))) T*)(size_t)addr;
-}
-template <class T>
-__device__ __forceinline__ void crs_lds_put(unsigned addr, T v) {
-  *(__attribute__((address_space(3))) T*)(size_t)addr = v;
-}
-
-// LDS layout over a supergather of S traces: 64 zero bytes; the trace table
-// (8 bytes a trace: window base, then qlo | nq << 16 | ungated << 31); the
-// batch starts (u16, S + 2: starts with bit 15 set when every trace of the
-// batch is ungated, the end, the batch count at index S + 1); the slab.
-__host__ __device__ constexpr int crs_lds_batches(int S) { return 64 + 8 * S; }
-__host__ __device__ constexpr int crs_lds_slab(int S) { return (64 + 8 * S + 2 * (S + 2) + 15) & ~15; }
-
-// Quads of trace s's window range over T0 in [T0lo, T0hi] and the box
-// [alo, ahi] x [blo, bhi] x [clo, chi]: pairs of samples elo .. ehi + W - 1
-// (samples: .. ehi + W) as the row's quads qlo .. qlo + nq - 1.
-template <int W, bool PAIRS>
-__device__ __forceinline__ void crs_range(const SearchArgs& a, const SlotF& s, float T0lo, float T0hi, float alo,
-                                          float ahi, float blo, float bhi, float clo, float chi, int& qlo, int& nq,
-                                          bool& full) {
-  const int nwin = max(a.ns - a.w, 0);
-  const float am0 = alo * s.m, am1 = ahi * s.m;
-  const float ulo = T0lo + fminf(am0, am1), uhi = T0hi + fmaxf(am0, am1);
-  const float u2lo = ulo * ulo, u2hi = uhi * uhi;
-  const float usqmin = (ulo <= 0.f && uhi >= 0.f) ? 0.f : fminf(u2lo, u2hi);
-  const float usqmax = fmaxf(u2lo, u2hi);
-  const float bm0 = blo * s.m2, bm1 = bhi * s.m2;
-  const float ch0 = clo * s.h, ch1 = chi * s.h;
-  const float ttlo = usqmin + fminf(bm0, bm1) + fminf(ch0, ch1);
-  const float tthi = usqmax + fmaxf(bm0, bm1) + fmaxf(ch0, ch1);
-  const int elo_b = (int)sqrtf(fmaxf(ttlo, 0.f)) - a.tau - 2;
-  const int ehi_b = (int)sqrtf(fmaxf(tthi, 0.f)) - a.tau + 2;
-  // every window of the span and box valid: no evaluation is gated, and
-  // t^2 >= (tau + 2)^2 > 0 for all of them
-  full = elo_b >= 0 && ehi_b <= nwin - 1;
-  const int elo = max(elo_b, 0), ehi = min(ehi_b, nwin - 1);
-  qlo = 0;
-  nq = 0;
-  if (ehi >= elo) {
-    qlo = elo >> 2;
-    nq = ((ehi + W - (PAIRS ? 1 : 0)) >> 2) - qlo + 1;
-  }
-}
-}  // namespace
 
 template <int W, int NC, int NW, bool PAIRS>
 __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
   // every rounding below is explicit (fmaf; plain operators under this pragma)
 #pragma clang fp contract(off)
   constexpr int NT = 64 * NW;
-  constexpr unsigned SB = PAIRS ? 8u : 4u, QB = 4u * SB;
-  constexpr int QSH = PAIRS ? 5 : 4;
   const WorkItem wi = a.work[blockIdx.x];
   if (wi.cdp < 0) return;  // the whole workgroup
   const int cdp = wi.cdp;
@@ -126,89 +76,22 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
   const float ea0 = fmaf(a.finc_a, (float)ia0, a.fa0), ea1 = fmaf(a.finc_a, (float)ia1, a.fa0);
   const float eb0 = fmaf(a.finc_b, (float)ib0, a.fb0), eb1 = fmaf(a.finc_b, (float)ib1, a.fb0);
   const float ec0 = fmaf(a.finc_c, 0.f, a.fc0), ec1 = fmaf(a.finc_c, (float)(a.nc - 1), a.fc0);
-  const float alo = fminf(ea0, ea1), ahi = fmaxf(ea0, ea1);
-  const float blo = fminf(eb0, eb1), bhi = fmaxf(eb0, eb1);
-  const float clo = fminf(ec0, ec1), chi = fmaxf(ec0, ec1);
+  const staged::Box box{{fminf(ea0, ea1), fminf(eb0, eb1), fminf(ec0, ec1)},
+                        {fmaxf(ea0, ea1), fmaxf(eb0, eb1), fmaxf(ec0, ec1)}};
   // the whole 64-t0 span, past ns for the last tile, so that every lane's
   // windows lie in the staged ranges
   const float T0lo = (float)(wi.tile * 64), T0hi = (float)(wi.tile * 64 + 63);
   const float T0 = (float)(wi.tile * 64 + lane);
   const float T0sq = T0 * T0;
   const float K1 = (2.f * T0) * pa, K2 = fmaf(pa, pa, pb);
-
-  const unsigned tab = 64u, bat = (unsigned)crs_lds_batches(S), slab = (unsigned)crs_lds_slab(S);
-  const int capq = (a.lds_bytes - (int)slab) >> QSH;
   const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
 
-  // ---- plan (once): each trace's quads, then the batches and window bases
-  if (tid < 16) crs_lds_put<unsigned>(4u * tid, 0u);  // gated windows read these zeros
-  for (int k = tid; k < S; k += NT) {
-    int qlo, nq;
-    bool full;
-    crs_range<W, PAIRS>(a, sl[k], T0lo, T0hi, alo, ahi, blo, bhi, clo, chi, qlo, nq, full);
-    crs_lds_put<unsigned>(tab + 8u * k + 4u, (unsigned)qlo | ((unsigned)nq << 16) | (full ? 0x80000000u : 0u));
-  }
-  __syncthreads();
-  if (tid < 64) {
-    // one wave: the next up to 64 traces whose quads fit the slab (the
-    // inclusive sums are nondecreasing, so those that fit are a prefix)
-    int nb = 0;
-    for (int kb = 0; kb < S;) {
-      const int k = kb + lane;
-      const unsigned e = k < S ? crs_lds_at<unsigned>(tab + 8u * k + 4u) : 0u;
-      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
-      int x = nq;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-      }
-      int K = __builtin_amdgcn_readfirstlane(__popcll(__builtin_amdgcn_ballot_w64(k < S && x <= capq)));
-      K = K > 0 ? K : 1;  // the host sizes the slab for any single trace
-      if (lane < K) crs_lds_put<unsigned>(tab + 8u * k, slab + QB * (unsigned)(x - nq - qlo) - SB * (unsigned)a.tau);
-      const bool gated = __builtin_amdgcn_ballot_w64(lane < K && !(e >> 31)) != 0ull;
-      if (lane == 0) crs_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)(kb | (gated ? 0 : 0x8000)));
-      nb++;
-      kb += K;
-    }
-    if (lane == 0) {
-      crs_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)S);
-      crs_lds_put<unsigned short>(bat + 2u * (S + 1), (unsigned short)nb);
-    }
-  }
-  __syncthreads();
-  const int nbatch = __builtin_amdgcn_readfirstlane((int)crs_lds_at<unsigned short>(bat + 2u * (S + 1)));
-  auto batch_at = [&](int b) { return __builtin_amdgcn_readfirstlane((int)crs_lds_at<unsigned short>(bat + 2u * b)); };
-
-  // Stage batch [kb, ke): trace kb + j by the NW threads j NW .. j NW + NW
-  // - 1, thread r taking the trace's quads r, r + NW, ...; global loads
-  // through per-lane row pointers (quads up to the one holding sample ns - 1
-  // and one sample more: inside the row's padding, stride >= ns + 8).
+  // ---- plan (once): the batches and window bases
+  const int nbatch = staged::plan<W, NT, PAIRS>(a, sl, S, T0lo, T0hi, box);
   auto stage = [&](int kb, int ke) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    typedef const __attribute__((address_space(1))) float GF;
-    typedef const __attribute__((address_space(1))) f4 GF4;
-    const int j = tid / NW, r = tid % NW;
-    if (j < ke - kb) {
-      const int k = kb + j;
-      const unsigned base = crs_lds_at<unsigned>(tab + 8u * k), e = crs_lds_at<unsigned>(tab + 8u * k + 4u);
-      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
-      // base = slab + QB (qoff - qlo) - SB tau
-      const int qoff = ((int)(base - slab + SB * (unsigned)a.tau) >> QSH) + qlo;
-      GF* row = (GF*)((const char*)a.samples + (int64_t)sl[k].row * a.stride * 4);
-      for (int q = r; q < nq; q += NW) {
-        GF* p = row + 4 * (qlo + q);
-        const f4 v = *(GF4*)p;
-        const unsigned dst = slab + QB * (unsigned)(qoff + q);
-        if constexpr (PAIRS) {
-          const float nx = p[4];
-          crs_lds_put<f4>(dst, f4{v.x, v.y - v.x, v.y, v.z - v.y});
-          crs_lds_put<f4>(dst + 16u, f4{v.z, v.w - v.z, v.w, nx - v.w});
-        } else {
-          crs_lds_put<f4>(dst, v);
-        }
-      }
-    }
+    staged::stage<NW, PAIRS>(a, S, kb, ke, [&](int k) {
+      return (const float*)((const char*)a.samples + (int64_t)sl[k].row * a.stride * 4);
+    });
   };
 
   float num[NC][W], den[NC];
@@ -226,7 +109,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
       typedef float f2 __attribute__((ext_vector_type(2)));
       f2 pr[W];
 #pragma unroll
-      for (int q = 0; q < W; q++) pr[q] = crs_lds_at<f2>(addr + 8u * q);
+      for (int q = 0; q < W; q++) pr[q] = staged::at<f2>(addr + 8u * q);
 #pragma unroll
       for (int q = 0; q < W; q++) {
         const float v = fmaf(pr[q].y, x, pr[q].x);
@@ -236,7 +119,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
     } else {
       float s[W + 1];
 #pragma unroll
-      for (int q = 0; q <= W; q++) s[q] = crs_lds_at<float>(addr + 4u * q);
+      for (int q = 0; q <= W; q++) s[q] = staged::at<float>(addr + 4u * q);
 #pragma unroll
       for (int q = 0; q < W; q++) {
         const float v = fmaf(s[q + 1] - s[q], x, s[q]);
@@ -250,7 +133,7 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
   auto run_batch = [&](int kb, int ke, bool ungated) {
     typedef const __attribute__((address_space(4))) SlotF CSlot;
     const int K = ke - kb;
-    const unsigned mybase = lane < K ? crs_lds_at<unsigned>(tab + 8u * (unsigned)(kb + lane)) : 0u;
+    const unsigned mybase = lane < K ? staged::at<unsigned>(staged::kTable + 8u * (unsigned)(kb + lane)) : 0u;
     CSlot* sk = (CSlot*)(size_t)(sl + kb);
     for (int j = 0; j < K; j++) {
       const float m = sk[j].m, m2 = sk[j].m2, h = sk[j].h;
@@ -283,10 +166,10 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
   if (nbatch == 1) {
     stage(0, S);
     __syncthreads();
-    if (wact) run_batch(0, S, (batch_at(0) & 0x8000) != 0);
+    if (wact) run_batch(0, S, (staged::batch_at(S, 0) & 0x8000) != 0);
   } else {
     for (int b = 0; b < nbatch; b++) {
-      const int vb = batch_at(b), kb = vb & 0x7fff, ke = batch_at(b + 1) & 0x7fff;
+      const int vb = staged::batch_at(S, b), kb = vb & 0x7fff, ke = staged::batch_at(S, b + 1) & 0x7fff;
       __syncthreads();  // every wave is done with the previous batch's slab
       stage(kb, ke);
       __syncthreads();
@@ -314,13 +197,13 @@ __global__ __launch_bounds__(64 * NW) void k_search_crs_lds(SearchArgs a) {
     }
   }
   __syncthreads();  // every wave is done with the tables and slab
-  crs_lds_put<unsigned long long>(64u + 8u * (unsigned)(wave * 64 + lane), best);
+  staged::put<unsigned long long>(64u + 8u * (unsigned)(wave * 64 + lane), best);
   __syncthreads();
   if (wave == 0) {
-    unsigned long long mx = crs_lds_at<unsigned long long>(64u + 8u * (unsigned)lane);
+    unsigned long long mx = staged::at<unsigned long long>(64u + 8u * (unsigned)lane);
 #pragma unroll
     for (int q = 1; q < NW; q++) {
-      const unsigned long long v = crs_lds_at<unsigned long long>(64u + 8u * (unsigned)(q * 64 + lane));
+      const unsigned long long v = staged::at<unsigned long long>(64u + 8u * (unsigned)(q * 64 + lane));
       mx = v > mx ? v : mx;
     }
     const int t0 = wi.tile * 64 + lane;
@@ -332,7 +215,7 @@ int crs_lds_min_bytes(int ns, int smax, int pairs, int waves) {
   // the plan of the longest supergather and the largest single-trace range
   // (every valid start plus the window, as quads, one more for the
   // alignment); the waves' keys reuse the space from byte 64
-  return std::max(crs_lds_slab(smax) + (pairs ? 32 : 16) * ((ns + 3) / 4 + 2), 64 + 8 * 64 * waves);
+  return std::max(staged::min_bytes(ns, smax, pairs != 0), 64 + 8 * 64 * waves);
 }
 
 int crs_lds_c_per_wave(int nc, int cpw) {

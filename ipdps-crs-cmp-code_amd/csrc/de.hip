@@ -16,6 +16,7 @@
 
 #include "device_common.hpp"
 #include "launch.hpp"
+#include "staged.hpp"
 
 namespace sb {
 
@@ -358,58 +359,6 @@ __global__ __launch_bounds__(256) void k_crs_de(SearchArgs a, DeArgs d, LD ld) {
 // zero either way), so both kernels produce the same bits: same fitness,
 // same selections, same outputs.  FP32 rows only (16-bit stores run on their
 // restored FP32 copy, or on k_crs_de).
-namespace {
-template <class T>
-__device__ __forceinline__ T de_lds_at(unsigned addr) {
-  return *(__attribute__((address_space(3))) T*)(size_t)addr;
-}
-template <class T>
-__device__ __forceinline__ void de_lds_put(unsigned addr, T v) {
-  *(__attribute__((address_space(3))) T*)(size_t)addr = v;
-}
-}  // namespace
-
-// LDS layout of a workgroup over a supergather of S traces: 64 zero bytes;
-// the trace table (8 bytes a trace: the window base, then qlo | nq << 16 |
-// ungated << 31); the batch starts (u16, S + 2: starts with bit 15 set when
-// every trace of the batch is ungated, the end, and the batch count at index
-// S + 1); the slab of 32-byte quads of pairs.
-__host__ __device__ constexpr int de_lds_table(int S) { return 64; }
-__host__ __device__ constexpr int de_lds_batches(int S) { return 64 + 8 * S; }
-__host__ __device__ constexpr int de_lds_slab(int S) { return (64 + 8 * S + 2 * (S + 2) + 15) & ~15; }
-
-// Quads of trace k's window range over T0 in [T0lo, T0hi] and the box:
-// pairs of samples elo .. ehi + W - 1 as the row's quads qlo .. qlo + nq - 1
-// (nq = 0: no window of the span and box is valid).
-template <int W, bool PAIRS>
-__device__ __forceinline__ void de_range(const SearchArgs& a, const DeArgs& d, const SlotF& s, float T0lo,
-                                         float T0hi, int& qlo, int& nq, bool& full) {
-  const int nwin = max(a.ns - a.w, 0);
-  const float am0 = d.lo[0] * s.m, am1 = d.hi[0] * s.m;
-  const float ulo = T0lo + fminf(am0, am1), uhi = T0hi + fmaxf(am0, am1);
-  const float u2lo = ulo * ulo, u2hi = uhi * uhi;
-  const float usqmin = (ulo <= 0.f && uhi >= 0.f) ? 0.f : fminf(u2lo, u2hi);
-  const float usqmax = fmaxf(u2lo, u2hi);
-  const float bm0 = d.lo[1] * s.m2, bm1 = d.hi[1] * s.m2;
-  const float ch0 = d.lo[2] * s.h, ch1 = d.hi[2] * s.h;
-  const float ttlo = usqmin + fminf(bm0, bm1) + fminf(ch0, ch1);
-  const float tthi = usqmax + fmaxf(bm0, bm1) + fmaxf(ch0, ch1);
-  const int elo_b = (int)sqrtf(fmaxf(ttlo, 0.f)) - a.tau - 2;
-  const int ehi_b = (int)sqrtf(fmaxf(tthi, 0.f)) - a.tau + 2;
-  // every window of the span and box valid: no candidate is gated, and
-  // t^2 >= (tau + 2)^2 > 0 for all of them
-  full = elo_b >= 0 && ehi_b <= nwin - 1;
-  const int elo = max(elo_b, 0), ehi = min(ehi_b, nwin - 1);
-  qlo = 0;
-  nq = 0;
-  if (ehi >= elo) {
-    // pairs: samples elo .. ehi + W - 1 with their differences; samples:
-    // elo .. ehi + W
-    qlo = elo >> 2;
-    nq = ((ehi + W - (PAIRS ? 1 : 0)) >> 2) - qlo + 1;
-  }
-}
-
 template <int W, int NW, bool PAIRS>
 __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) {
   // as k_crs_de: the initial draws and the mutation are separate roundings
@@ -434,87 +383,15 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
   const int k0 = a.slot_off[cdp];
   const int S = a.slot_off[cdp + 1] - k0;
   const float T0lo = (float)t0first, T0hi = (float)min(t0first + per_block - 1, a.ns - 1);
-  const unsigned tab = (unsigned)de_lds_table(S), bat = (unsigned)de_lds_batches(S), slab = (unsigned)de_lds_slab(S);
-  // slab bytes per staged sample and per quad
-  constexpr unsigned SB = PAIRS ? 8u : 4u, QB = 4u * SB;
-  constexpr int QSH = PAIRS ? 5 : 4;
-  const int capq = (a.lds_bytes - (int)slab) >> QSH;
   const unsigned nwin = (unsigned)max(a.ns - a.w, 0);
   const SlotF* __restrict__ sl = a.slots + k0;
 
-  // ---- plan (once): each trace's quads, then the batches and window bases
-  if (tid < 16) de_lds_put<unsigned>(4u * tid, 0u);  // gated windows read these zeros
-  for (int k = tid; k < S; k += NT) {
-    int qlo, nq;
-    bool full;
-    de_range<W, PAIRS>(a, d, sl[k], T0lo, T0hi, qlo, nq, full);
-    de_lds_put<unsigned>(tab + 8u * k + 4u, (unsigned)qlo | ((unsigned)nq << 16) | (full ? 0x80000000u : 0u));
-  }
-  __syncthreads();
-  if (tid < 64) {
-    // one wave: the next up to 64 traces whose quads fit the slab (the
-    // inclusive sums are nondecreasing, so those that fit are a prefix)
-    int nb = 0;
-    for (int kb = 0; kb < S;) {
-      const int k = kb + lane;
-      const unsigned e = k < S ? de_lds_at<unsigned>(tab + 8u * k + 4u) : 0u;
-      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
-      int x = nq;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-      }
-      int K = __builtin_amdgcn_readfirstlane(__popcll(__builtin_amdgcn_ballot_w64(k < S && x <= capq)));
-      K = K > 0 ? K : 1;  // the host sizes the slab for any single trace
-      if (lane < K) de_lds_put<unsigned>(tab + 8u * k, slab + QB * (unsigned)(x - nq - qlo) - SB * (unsigned)a.tau);
-      // bit 15: every trace of the batch has only valid windows
-      const bool gated = __builtin_amdgcn_ballot_w64(lane < K && !(e >> 31)) != 0ull;
-      if (lane == 0) de_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)(kb | (gated ? 0 : 0x8000)));
-      nb++;
-      kb += K;
-    }
-    if (lane == 0) {
-      de_lds_put<unsigned short>(bat + 2u * nb, (unsigned short)S);
-      de_lds_put<unsigned short>(bat + 2u * (S + 1), (unsigned short)nb);
-    }
-  }
-  __syncthreads();
-  const int nbatch = __builtin_amdgcn_readfirstlane((int)de_lds_at<unsigned short>(bat + 2u * (S + 1)));
-
-  // Stage batch [kb, ke): trace kb + j by the NW threads j NW .. j NW + NW
-  // - 1, thread r taking the trace's quads r, r + NW, ...
+  // ---- plan (once): the batches and window bases
+  const staged::Box box{{d.lo[0], d.lo[1], d.lo[2]}, {d.hi[0], d.hi[1], d.hi[2]}};
+  const int nbatch = staged::plan<W, NT, PAIRS>(a, sl, S, T0lo, T0hi, box);
   auto stage = [&](int kb, int ke) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    const int j = tid / NW, r = tid % NW;
-    if (j < ke - kb) {
-      const int k = kb + j;
-      const unsigned base = de_lds_at<unsigned>(tab + 8u * k), e = de_lds_at<unsigned>(tab + 8u * k + 4u);
-      const int nq = (int)((e >> 16) & 0x7fffu), qlo = (int)(e & 0xffffu);
-      // base = slab + QB (qoff - qlo) - SB tau
-      const int qoff = ((int)(base - slab + SB * (unsigned)a.tau) >> QSH) + qlo;
-      // per-lane row pointers (a buffer resource would have to be uniform):
-      // quads up to the one holding sample ns - 1, then one sample more,
-      // inside the row's padding (stride >= ns + 8)
-      typedef const __attribute__((address_space(1))) float GF;
-      typedef const __attribute__((address_space(1))) f4 GF4;
-      GF* row = (GF*)(size_t)sl[k].addr;
-      for (int q = r; q < nq; q += NW) {
-        GF* p = row + 4 * (qlo + q);
-        const f4 v = *(GF4*)p;
-        const unsigned dst = slab + QB * (unsigned)(qoff + q);
-        if constexpr (PAIRS) {
-          const float nx = p[4];
-          de_lds_put<f4>(dst, f4{v.x, v.y - v.x, v.y, v.z - v.y});
-          de_lds_put<f4>(dst + 16u, f4{v.z, v.w - v.z, v.w, nx - v.w});
-        } else {
-          de_lds_put<f4>(dst, v);
-        }
-      }
-    }
+    staged::stage<NW, PAIRS>(a, S, kb, ke, [&](int k) { return (const float*)(size_t)sl[k].addr; });
   };
-  // batch b's first trace (bit 15: the batch is ungated)
-  auto batch_at = [&](int b) { return __builtin_amdgcn_readfirstlane((int)de_lds_at<unsigned short>(bat + 2u * b)); };
 
   const float T0 = (float)t0c;
   const float T0sq = T0 * T0;
@@ -530,7 +407,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
       typedef float f2 __attribute__((ext_vector_type(2)));
       f2 pr[W];
 #pragma unroll
-      for (int q = 0; q < W; q++) pr[q] = de_lds_at<f2>(addr + 8u * q);
+      for (int q = 0; q < W; q++) pr[q] = staged::at<f2>(addr + 8u * q);
 #pragma unroll
       for (int q = 0; q < W; q++) {
         const float v = fmaf(pr[q].y, x, pr[q].x);
@@ -540,7 +417,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
     } else {
       float r[W + 1];
 #pragma unroll
-      for (int q = 0; q <= W; q++) r[q] = de_lds_at<float>(addr + 4u * q);
+      for (int q = 0; q <= W; q++) r[q] = staged::at<float>(addr + 4u * q);
 #pragma unroll
       for (int q = 0; q < W; q++) {
         const float v = fmaf(r[q + 1] - r[q], x, r[q]);
@@ -552,7 +429,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
   auto run_batch = [&](int kb, int ke, bool ungated, float K1, float K2, float pc) {
     typedef const __attribute__((address_space(4))) SlotF CSlot;
     const int K = ke - kb;
-    const unsigned mybase = lane < K ? de_lds_at<unsigned>(tab + 8u * (unsigned)(kb + lane)) : 0u;
+    const unsigned mybase = lane < K ? staged::at<unsigned>(staged::kTable + 8u * (unsigned)(kb + lane)) : 0u;
     CSlot* sk = (CSlot*)(size_t)(sl + kb);
     if (ungated) {
       // every window valid (the plan's bound): no test, no count per trace
@@ -583,7 +460,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
   };
   // Fitness of (pa, pb, pc) over the whole supergather.  A supergather that
   // is one batch is staged once for every generation.
-  const int b0 = batch_at(0);
+  const int b0 = staged::batch_at(S, 0);
   if (nbatch == 1) {
     stage(0, S);
     __syncthreads();
@@ -598,7 +475,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
       run_batch(0, S, (b0 & 0x8000) != 0, K1, K2, pc);
     } else {
       for (int b = 0; b < nbatch; b++) {
-        const int vb = batch_at(b), kb = vb & 0x7fff, ke = batch_at(b + 1) & 0x7fff;
+        const int vb = staged::batch_at(S, b), kb = vb & 0x7fff, ke = staged::batch_at(S, b + 1) & 0x7fff;
         __syncthreads();  // every lane is done with the previous batch's slab
         stage(kb, ke);
         __syncthreads();
@@ -690,10 +567,7 @@ __global__ __launch_bounds__(64 * NW) void k_crs_de_lds(SearchArgs a, DeArgs d) 
 }
 
 int de_lds_min_bytes(int ns, int smax, int pairs) {
-  // the table and batch starts of the longest supergather, then the largest
-  // single-trace range: every valid start plus the window, as quads, one
-  // more for the alignment
-  return de_lds_slab(smax) + (pairs ? 32 : 16) * ((ns + 3) / 4 + 2);
+  return staged::min_bytes(ns, smax, pairs != 0);
 }
 
 bool de_lds_supported(int w, int pop, int waves, int pairs, int lds_bytes, int ns, int smax) {
