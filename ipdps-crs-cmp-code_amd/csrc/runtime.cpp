@@ -224,6 +224,7 @@ extern "C" void sb_ctx_destroy(sb_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx_release_all(ctx);
   if (ctx->work_host) (void)hipHostFree(ctx->work_host);
+  for (void* p : ctx->work_host_retired) (void)hipHostFree(p);
   if (ctx->work_host_ev) (void)hipEventDestroy(ctx->work_host_ev);
   for (auto& ev : ctx->ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
@@ -597,11 +598,13 @@ int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, in
     const size_t nb = (size_t)n * sizeof(WorkItem);
     if (ctx->work_host_ev) HIPR(hipEventSynchronize(ctx->work_host_ev));
     if (nb > ctx->work_host_cap) {
-      if (ctx->work_host) (void)hipHostFree(ctx->work_host);
-      ctx->work_host = nullptr;
-      ctx->work_host_cap = 0;
-      HIPR(hipHostMalloc(&ctx->work_host, nb * 2, hipHostMallocDefault));
-      ctx->work_host_cap = nb * 2;
+      // at least doubles, from 1 MiB (43690 work items)
+      const size_t cap = std::max({nb * 2, ctx->work_host_cap * 2, size_t(1) << 20});
+      void* p = nullptr;
+      HIPR(hipHostMalloc(&p, cap, hipHostMallocDefault));
+      if (ctx->work_host) ctx->work_host_retired.push_back(ctx->work_host);
+      ctx->work_host = p;
+      ctx->work_host_cap = cap;
     }
     WorkItem* items = (WorkItem*)ctx->work_host;
     for (int x = 0; x < kXcd; x++) {

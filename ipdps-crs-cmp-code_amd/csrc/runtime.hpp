@@ -42,6 +42,9 @@ struct sb_ctx {
   std::string work_key;  // identifies the table in work_buf
   void* work_host = nullptr;  // pinned staging of the table upload
   size_t work_host_cap = 0;
+  // Outgrown staging buffers, freed with the ctx (hipHostFree waits for the
+  // device: other contexts' queued work would stall this one's search).
+  std::vector<void*> work_host_retired;
   hipEvent_t work_host_ev = nullptr;  // the last upload from work_host has completed
   int work_n = 0;
   unsigned long long gather_serial = 0;

@@ -50,6 +50,9 @@ constexpr int kSlots = 4;  // matches the 4 hardware queues a process gets by de
 constexpr double kSmallEvals = 5e8;
 // Device bytes each slot keeps cached between requests (outside the budget).
 constexpr size_t kSlotCache = size_t(512) << 20;
+// Pinned download staging per slot at start: 64 CDPs x 5461 samples of
+// results (24 bytes each).
+constexpr size_t kPinStart = size_t(8) << 20;
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -169,19 +172,37 @@ struct sb_server {
   sb_ctx* res_ctx = nullptr;  // owns the resident gathers
   std::vector<void*> pin;     // per slot: pinned download staging
   std::vector<size_t> pin_cap;
+  // Outgrown staging buffers, freed with the server: hipHostFree waits for
+  // the device, which would stall the slot behind the other slots' searches
+  // for milliseconds.
+  std::vector<void*> pin_retired;
+  std::mutex pin_mu;  // guards pin_retired
+
+  // Grows slot `slot`'s staging to at least `bytes` (at least doubling).
+  int grow_pin(int slot, size_t bytes) {
+    size_t cap = std::max(bytes, 2 * pin_cap[slot]);
+    cap = (cap + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    void* p = nullptr;
+    if (hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess)
+      return fail(SB_ERR_HIP, "pinned staging allocation failed");
+    if (pin[slot]) {
+      std::lock_guard<std::mutex> lk(pin_mu);
+      pin_retired.push_back(pin[slot]);
+    }
+    pin[slot] = p;
+    pin_cap[slot] = cap;
+    return SB_OK;
+  }
 
   // Downloads n = rows * ns results of `res` through the slot's pinned
   // buffer (a true DMA; pageable copies serialise through the runtime's
-  // staging) and hands out the six arrays.
+  // staging) and hands out the six arrays, valid until the slot's next
+  // download.
   int download(int slot, const sb_result* res, size_t n, sb_host_result* h) {
     const size_t bytes = n * 24;
     if (bytes > pin_cap[slot]) {
-      if (pin[slot]) (void)hipHostFree(pin[slot]);
-      pin[slot] = nullptr;
-      pin_cap[slot] = 0;
-      if (hipHostMalloc(&pin[slot], bytes, hipHostMallocDefault) != hipSuccess)
-        return fail(SB_ERR_HIP, "pinned staging allocation failed");
-      pin_cap[slot] = bytes;
+      const int rc = grow_pin(slot, bytes);
+      if (rc) return rc;
     }
     char* p = (char*)pin[slot];
     *h = sb_host_result{(int32_t*)p, (float*)(p + n * 4), (float*)(p + n * 8), (float*)(p + n * 12),
@@ -451,6 +472,16 @@ extern "C" int sb_server_create(int device, size_t hbm_budget, int max_batch, sb
   }
   s->pin.assign(kSlots, nullptr);
   s->pin_cap.assign(kSlots, 0);
+  for (int i = 0; i < kSlots; i++) {
+    const int rc = s->grow_pin(i, kPinStart);
+    if (rc) {
+      for (void* p : s->pin)
+        if (p) (void)hipHostFree(p);
+      for (auto* x : s->ctx) sb_ctx_destroy(x);
+      sb_ctx_destroy(s->res_ctx);
+      return rc;
+    }
+  }
   // Every slot ctx may use the whole budget; the server keeps the sum within it.
   size_t b = 0;
   sb_ctx_memory(s->ctx[0], nullptr, &b);
@@ -651,5 +682,6 @@ extern "C" void sb_server_destroy(sb_server* s) {
   sb_ctx_destroy(s->res_ctx);
   for (void* p : s->pin)
     if (p) (void)hipHostFree(p);
+  for (void* p : s->pin_retired) (void)hipHostFree(p);
   delete s;
 }
