@@ -393,11 +393,16 @@ class DeviceResult:
         self.h = C.c_void_p()
         check(lib().sb_result_create(ctx.h, ncdp, ns, C.byref(self.h)))
 
-    def download(self):
-        out = {k: np.zeros((self.ncdp, self.ns), np.int32 if k == "index" else np.float32)
-               for k in ("index", "a", "b", "c", "coher", "stack")}
-        hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float)
-                          for k in ("index", "a", "b", "c", "coher", "stack")])
+    def download(self, contiguous=False):
+        """The six result arrays.  contiguous: views of one host buffer laid
+        out as the device arrays are (the library then makes one copy)."""
+        keys = ("index", "a", "b", "c", "coher", "stack")
+        if contiguous:
+            buf = np.zeros((6, self.ncdp, self.ns), np.float32)
+            out = {k: buf[i].view(np.int32) if k == "index" else buf[i] for i, k in enumerate(keys)}
+        else:
+            out = {k: np.zeros((self.ncdp, self.ns), np.int32 if k == "index" else np.float32) for k in keys}
+        hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float) for k in keys])
         check(lib().sb_result_download(self.ctx.h, self.h, C.byref(hr)))
         return out
 
@@ -570,10 +575,9 @@ class Server:
 
     def wait(self, ticket, timeout_ms=-1.0):
         keep, ncdp, ns = self._keep[ticket]
-        out = {k: np.zeros((ncdp, ns), np.int32 if k == "index" else np.float32)
-               for k in ("index", "a", "b", "c", "coher", "stack")}
-        hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float)
-                          for k in ("index", "a", "b", "c", "coher", "stack")])
+        keys = ("index", "a", "b", "c", "coher", "stack")
+        out = {k: np.zeros((ncdp, ns), np.int32 if k == "index" else np.float32) for k in keys}
+        hr = HostResult(*[_ptr(out[k], C.c_int32 if k == "index" else C.c_float) for k in keys])
         st = RequestStats()
         rc = lib().sb_server_wait(self.h, ticket, timeout_ms, C.byref(hr), C.byref(st))
         self._keep.pop(ticket, None)

@@ -541,8 +541,18 @@ extern "C" int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_resul
     const void* d;
   } m[] = {{out->index, r->index}, {out->a, r->a},         {out->b, r->b},
            {out->c, r->c},         {out->coher, r->coher}, {out->stack, r->stack}};
-  for (auto& x : m)
-    if (x.h) HIPR(hipMemcpyAsync(x.h, x.d, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  // The device arrays are consecutive; host arrays laid out the same way
+  // (as the serving loop's staging is) take one copy instead of six.
+  bool one = true;
+  for (int i = 0; i < 6; i++)
+    one = one && m[i].h && (char*)m[i].h == (char*)m[0].h + n * 4 * i &&
+          (const char*)m[i].d == (const char*)m[0].d + n * 4 * i;
+  if (one) {
+    HIPR(hipMemcpyAsync(m[0].h, m[0].d, n * 24, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    for (auto& x : m)
+      if (x.h) HIPR(hipMemcpyAsync(x.h, x.d, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
   HIPR(hipStreamSynchronize(ctx->stream));
   return SB_OK;
 }

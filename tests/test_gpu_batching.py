@@ -81,6 +81,8 @@ def test_graph_replays_equal_eager(sb, gpu_ctx, persistent):
                         gr.launch()
                         got = r.download()
                         assert _same(got, want), (prm, rep)
+                        # one copy into a host buffer laid out as the device's
+                        assert _same(r.download(contiguous=True), want), (prm, rep)
                         t = gpu_ctx.timing()
                         assert t["total_ms"] > 0
                         # eager work on the same ctx between replays
