@@ -187,6 +187,30 @@ def test_resident_gathers_alone_and_in_the_mix(sb, gpu_ctx):
             assert np.array_equal(out[k], want[k]), (kind, cb, ce, k)
 
 
+def test_results_beyond_the_initial_staging(sb, gpu_ctx):
+    """Range requests whose results (24 bytes a (CDP, t0)) outgrow a slot's
+    initial 8 MiB of pinned download staging: 9.2-21.6 MB here, so the
+    staging grows (twice over on some slot) while requests run on the other
+    slots; every result equals the rows of the line searched alone."""
+    print("case: serving results of 0.1-21.6 MB", flush=True)
+    cdp_off, geom, samples = sb.synth(0, 300, (6, 10), 3001, 2000, 31, h_max=1500.0, nevents=4)
+    prm = sb.cmp_params(1.98e-7, 1.77e-6, 32, 0.002)
+    g = sb.Gather(gpu_ctx, cdp_off, geom, samples, 2000)
+    try:
+        ref = sb.search(gpu_ctx, g, prm)
+    finally:
+        g.close()
+    ranges = [(0, 128), (10, 300), (5, 7), (0, 300), (100, 250), (0, 300)]
+    with sb.Server(0, max_batch=300) as srv:
+        gid = srv.add_gather(cdp_off, geom, samples, 2000)
+        jobs = [(cb, ce, srv.submit_range(gid, cb, ce, prm)) for cb, ce in ranges]
+        outs = [(cb, ce, srv.wait(t)) for cb, ce, t in jobs]
+    for cb, ce, (out, st) in outs:
+        assert out is not None, st
+        for k in ("index", "a", "b", "c", "coher", "stack"):
+            assert np.array_equal(out[k], ref[k][cb:ce]), (cb, ce, k)
+
+
 def test_admission_deadline_sheds_load(sb, gpu_ctx):
     """With an admission deadline, a burst far above the GPU's capacity is
     shed: requests not admitted within the deadline fail with
