@@ -51,6 +51,20 @@ static size_t round_block(size_t bytes) {
   return (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
 }
 
+// Blocks of transient allocations (restored FP32 copies, sized per request)
+// above 32 MiB come in power-of-two classes, so that a cached block serves
+// every later copy of its class: sizes that vary per request would otherwise
+// miss the cache, allocate anew and push older blocks out (hipFree).
+size_t transient_block_size(size_t bytes) {
+  size_t want = round_block(bytes);
+  if (want > (size_t(32) << 20)) {
+    size_t c = size_t(32) << 20;
+    while (c < want) c <<= 1;
+    want = c;
+  }
+  return want;
+}
+
 // The free-event of a cached block goes back to the pool once the block is
 // reused or released (hipFree waits for the device anyway).
 static void drop_event(sb_ctx* ctx, hipEvent_t ev) {
@@ -73,7 +87,7 @@ static void release_cache(sb_ctx* ctx) {
 static int alloc_block(sb_ctx* ctx, size_t bytes, void** p, bool transient) {
   *p = nullptr;
   if (bytes == 0) return SB_OK;
-  const size_t want = round_block(bytes);
+  const size_t want = transient ? transient_block_size(bytes) : round_block(bytes);
   std::lock_guard<std::mutex> lk(ctx->mem_mu);
   auto it = ctx->cache.lower_bound(want);
   if (it != ctx->cache.end() && it->first <= 2 * want) {

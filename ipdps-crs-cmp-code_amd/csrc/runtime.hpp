@@ -118,6 +118,8 @@ void ctx_free(sb_ctx* ctx, void* p);
 void ctx_release_all(sb_ctx* ctx);
 // Frees cached blocks, largest first, until at most keep bytes stay cached.
 void ctx_trim(sb_ctx* ctx, size_t keep);
+// Device bytes a transient allocation of `bytes` takes (its block class).
+size_t transient_block_size(size_t bytes);
 int hip_fail(hipError_t e, const char* what);
 int build_work(sb_ctx* ctx, const std::vector<int64_t>& cost, int cb, int ce, int ntiles, int npar, int pchunk,
                int* nblocks, const std::string& key);

@@ -48,8 +48,10 @@ constexpr int kSlots = 4;  // matches the 4 hardware queues a process gets by de
 // a quarter of a millisecond of GPU time, e.g. one 1-CDP panel), so
 // interactive panels never queue behind a longer search.
 constexpr double kSmallEvals = 5e8;
-// Device bytes each slot keeps cached between requests (outside the budget).
-constexpr size_t kSlotCache = size_t(512) << 20;
+// Device bytes each slot keeps cached between requests (outside the budget):
+// enough for a few restored copies of the largest classes, so that 16-bit
+// requests reuse them instead of freeing (hipFree waits for the device).
+constexpr size_t kSlotCache = size_t(2) << 30;
 // Pinned download staging per slot at start: 64 CDPs x 5461 samples of
 // results (24 bytes each).
 constexpr size_t kPinStart = size_t(8) << 20;
@@ -110,11 +112,12 @@ size_t footprint(const sb_request& r, int ntr) {
 }
 
 // A 16-bit store's search restores the rows it reads to FP32 in a transient
-// copy (SB_OPT_RESTORE_FP32, on for the slot contexts): nrows rows.
+// copy (SB_OPT_RESTORE_FP32, on for the slot contexts): nrows rows, in the
+// allocator's block class.
 size_t restore_bytes(int store, int ns, int64_t nrows) {
   if (store == SB_STORE_F32) return 0;
   const int64_t stride = ((int64_t)ns + 8 + 63) / 64 * 64;
-  return (size_t)nrows * stride * 4 + 256;
+  return transient_block_size((size_t)nrows * stride * 4 + 256);
 }
 
 // Range request on a resident gather: the results, the work table and a
