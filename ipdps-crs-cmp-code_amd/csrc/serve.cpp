@@ -48,6 +48,9 @@ constexpr int kSlots = 4;  // matches the 4 hardware queues a process gets by de
 // a quarter of a millisecond of GPU time, e.g. one 1-CDP panel), so
 // interactive panels never queue behind a longer search.
 constexpr double kSmallEvals = 5e8;
+// Slot 1 takes the oldest small request instead once one has waited this
+// long (slot 0 is behind), otherwise it serves like slots 2 and 3.
+constexpr double kSmallHelpMs = 4.0;
 // Device bytes each slot keeps cached between requests (outside the budget):
 // enough for a few restored copies of the largest classes, so that 16-bit
 // requests reuse them instead of freeing (hipFree waits for the device).
@@ -381,8 +384,18 @@ void sb_server::worker(int slot) {
       // Slots 1..: the oldest request that fits the free budget.  Slot 0:
       // the smallest small request that fits (shortest job first, so 1-CDP
       // panels do not queue behind larger small ones), unless one waited a
-      // second.  A request alone larger than the whole budget is rejected.
+      // second.  Slot 1 first: the oldest small request that fits once it
+      // has waited kSmallHelpMs.  A request alone larger than the whole
+      // budget is rejected.
       const double now = now_ms(t0);
+      if (slot == 1) {
+        for (auto& q : pending) {
+          if (q->evals > kSmallEvals || (q->bytes <= budget && reserved + q->bytes > budget)) continue;
+          if (now - q->t_submit <= kSmallHelpMs) break;  // the rest are newer
+          head = q;
+          return true;
+        }
+      }
       std::shared_ptr<Req> best;
       for (auto& q : pending) {
         if (slot == 0 && q->evals > kSmallEvals) continue;
