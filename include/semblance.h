@@ -202,7 +202,9 @@ typedef struct {
 typedef struct sb_result sb_result; /* device-resident result arrays */
 int sb_result_create(sb_ctx* ctx, int ncdp, int ns, sb_result** out);
 void sb_result_destroy(sb_result* r);
-/* Copies a result to host arrays (any pointer may be NULL to skip it). */
+/* Copies a result to host arrays (any pointer may be NULL to skip it).  Host
+ * arrays that are consecutive in the order index, a, b, c, coher, stack
+ * (ncdp * ns elements each) are filled by one copy; pinned ones copy by DMA. */
 int sb_result_download(sb_ctx* ctx, const sb_result* r, sb_host_result* out);
 
 /* CMP search: replaces compute_semblances + redux_semblances,
