@@ -631,6 +631,13 @@ def main():
                     help="time only the CRS grid configuration (for profiling): --crs-cdps, --crs-lds")
     ap.add_argument("--crs-cdps", type=int, default=500, help="with --crs-only: CDPs of the CRS line")
     ap.add_argument("--crs-lds", type=int, default=-1, help="with --crs-only: SB_OPT_CRS_LDS (0: global memory)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving workload once (for profiling): --serve-rate, --serve-deadline, "
+                         "--serve-store")
+    ap.add_argument("--serve-rate", type=float, default=1000.0,
+                    help="with --serve-only: mixed requests/s offered (plus 100/s 1-CDP probes)")
+    ap.add_argument("--serve-deadline", type=float, default=0.0, help="with --serve-only: admission deadline, ms")
+    ap.add_argument("--serve-store", default="f32", choices=("f32", "f16"), help="with --serve-only: line store")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the search shards by CDP range "
@@ -644,6 +651,11 @@ def main():
             return
         print("bench.py: --crs-only times the GPU only; pass --no-cpu", file=sys.stderr)
         sys.exit(2)
+    if args.serve_only:
+        r = serving_bench(sb, serving_lines(sb), seconds=2.0, rate=args.serve_rate,
+                          deadline_ms=args.serve_deadline, store=0 if args.serve_store == "f32" else 1)
+        print(json.dumps(r), flush=True)
+        return
     if args.de_only:
         orc = _load("sb_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
         if args.no_cpu:
