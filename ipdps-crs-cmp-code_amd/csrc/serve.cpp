@@ -390,7 +390,9 @@ void sb_server::worker(int slot) {
       const double now = now_ms(t0);
       if (slot == 1) {
         for (auto& q : pending) {
-          if (q->evals > kSmallEvals || (q->bytes <= budget && reserved + q->bytes > budget)) continue;
+          const bool fits = q->bytes > budget || reserved + q->bytes <= budget;
+          if (!fits && now - q->t_submit > 1000.0) break;  // nothing overtakes it (as below)
+          if (q->evals > kSmallEvals || !fits) continue;
           if (now - q->t_submit <= kSmallHelpMs) break;  // the rest are newer
           head = q;
           return true;
